@@ -1,0 +1,106 @@
+// ref_driver.cpp -- drives the reference library through its public API only
+// (src/trottersuzuki.h), the way the reference's examples do.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile against the reference
+// sources (never copied into this repository) into oracle/_ref/.  Used to write
+// the golden vectors under tests/golden/ and, in timing mode, as the CPU
+// baseline of bench.py.
+//
+// Case file (little-endian, all tile arrays in the reference layout W*H incl. halo):
+//   int32  magic 0x54534d49, version 1
+//   int32  Nx, Ny, periodic_x, periodic_y, n_steps, imag_time, n_warmup, time_mode
+//   double Lx, Ly, dt, mass, g, g_lhy
+//   double psi_re[W*H], psi_im[W*H], V[W*H]
+// Output file:
+//   double psi_re[W*H], psi_im[W*H]
+//   double energies[6]  = total, kinetic, potential, intra, LHY, squared norm (Solver getters)
+//   double expect[10]   = State: norm2, X, X^2, Y, Y^2, P_x, P_x^2, P_y, P_y^2, L_z
+// In time mode the driver runs n_warmup steps, then times evolve(n_steps) with
+// steady_clock and prints one JSON line; no output file is written.
+#include "trottersuzuki.h"
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static void die(const char *m) { std::fprintf(stderr, "ref_driver: %s\n", m); std::exit(2); }
+
+template <class T> static void rd(FILE *f, T *p, size_t n) {
+    if (std::fread(p, sizeof(T), n, f) != n) die("short read");
+}
+
+int main(int argc, char **argv) {
+    if (argc < 2) die("usage: ref_driver case.bin [out.bin]");
+    FILE *f = std::fopen(argv[1], "rb");
+    if (!f) die("cannot open case file");
+    int32_t hdr[10];
+    rd(f, hdr, 10);
+    if (hdr[0] != 0x54534d49 || hdr[1] != 1) die("bad case header");
+    const int Nx = hdr[2], Ny = hdr[3], px = hdr[4], py = hdr[5];
+    const int n_steps = hdr[6], imag = hdr[7], n_warm = hdr[8], time_mode = hdr[9];
+    double par[6];
+    rd(f, par, 6);
+    const double Lx = par[0], Ly = par[1], dt = par[2], mass = par[3], g = par[4], g_lhy = par[5];
+
+    Lattice2D grid(Nx, Lx, Ny, Ly, px != 0, py != 0);
+    const size_t n = (size_t)grid.dim_x * (size_t)grid.dim_y;
+    std::vector<double> re(n), im(n), v(n);
+    rd(f, re.data(), n);
+    rd(f, im.data(), n);
+    rd(f, v.data(), n);
+    std::fclose(f);
+
+    State state(&grid, 0, re.data(), im.data());
+    Potential pot(&grid, v.data());
+    Hamiltonian ham(&grid, &pot, mass, g, g_lhy);
+    Solver solver(&grid, &state, &ham, dt, "cpu");
+
+    if (time_mode) {
+        if (n_warm > 0) solver.evolve(n_warm, imag != 0);
+        auto t0 = std::chrono::steady_clock::now();
+        solver.evolve(n_steps, imag != 0);
+        auto t1 = std::chrono::steady_clock::now();
+        const double s = std::chrono::duration<double>(t1 - t0).count();
+        int threads = 1;
+#ifdef _OPENMP
+        threads = omp_get_max_threads();
+#endif
+        std::printf("{\"seconds\": %.9g, \"steps\": %d, \"dim_x\": %d, \"dim_y\": %d, \"threads\": %d}\n",
+                    s, n_steps, grid.dim_x, grid.dim_y, threads);
+        return 0;
+    }
+
+    solver.evolve(n_steps, imag != 0);
+    double e[6];
+    e[0] = solver.get_total_energy();
+    e[1] = solver.get_kinetic_energy();
+    e[2] = solver.get_potential_energy();
+    e[3] = solver.get_intra_species_energy();
+    e[4] = solver.get_LeeHuangYang_energy();
+    e[5] = solver.get_squared_norm();
+    double x[10];
+    x[0] = state.get_squared_norm();
+    x[1] = state.get_mean_x();
+    x[2] = state.get_mean_xx();
+    x[3] = state.get_mean_y();
+    x[4] = state.get_mean_yy();
+    x[5] = state.get_mean_px();
+    x[6] = state.get_mean_pxpx();
+    x[7] = state.get_mean_py();
+    x[8] = state.get_mean_pypy();
+    x[9] = state.get_mean_angular_momentum();
+    if (argc < 3) die("missing output path");
+    FILE *o = std::fopen(argv[2], "wb");
+    if (!o) die("cannot open output");
+    std::fwrite(re.data(), sizeof(double), n, o);
+    std::fwrite(im.data(), sizeof(double), n, o);
+    std::fwrite(e, sizeof(double), 6, o);
+    std::fwrite(x, sizeof(double), 10, o);
+    std::fclose(o);
+    return 0;
+}
