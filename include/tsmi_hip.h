@@ -1,0 +1,127 @@
+/*
+ * tsmi_hip.h -- C ABI of the MI355X (gfx950) Trotter-Suzuki step library
+ * (libtsmi_hip.so).
+ *
+ * This is the drop-in boundary for the reference's kernel interface
+ * ITrotterKernel (src/trottersuzuki.h:502-520): a host-side kernel object
+ * (src/kernel.h:66-151 for the CPU one) becomes an opaque context that owns the
+ * device copies of the wave function, of the potential factor E = exp(-i dt V)
+ * and of the raw potential.  Only plain C types cross the boundary: pointers,
+ * sizes, doubles and int32.  Every call returns TSMI_OK (0) or an error code;
+ * tsmi_last_error() gives the message of the calling thread's last failure
+ * (the reference reports errors through my_abort -> std::runtime_error,
+ * src/common.cpp:104-115; the C++ wrapper turns a non-zero status into that).
+ *
+ * Arrays passed by the host use the reference tile layout: row-major W x H
+ * doubles, idx = y*W + x, with W = dim_x + 8*periodic_x and H = dim_y +
+ * 8*periodic_y (src/model.cpp:140-157, src/common.cpp:83-102).
+ */
+#ifndef TSMI_HIP_H
+#define TSMI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TSMI_OK = 0,
+    TSMI_E_INVALID = 1,      /* bad argument or state                         */
+    TSMI_E_HIP = 2,          /* HIP runtime error (message has the HIP string) */
+    TSMI_E_OOM = 3,          /* device allocation failed                      */
+    TSMI_E_UNSUPPORTED = 4   /* feature outside this library's path           */
+};
+
+typedef struct tsmi_ctx tsmi_ctx;
+
+/* Everything CPUBlock's constructor reads from Lattice, Hamiltonian and the
+ * Solver (src/cpukernel.cpp:172-272, src/kernel.h:68-70), for one component. */
+typedef struct {
+    int32_t dim_x, dim_y;          /* inner lattice points (global_no_halo_dim_x/y)      */
+    int32_t periodic_x, periodic_y;
+    int32_t halo_x, halo_y;        /* lattice halo widths; 4 on this path                */
+    int32_t imag_time;             /* 0: real time, 1: imaginary time                    */
+    int32_t precision;             /* 64: FP64 throughout; 32: FP32 bulk, FP64 reductions */
+    int32_t tile_rows;             /* rows per wavefront segment; 0 chooses              */
+    int32_t reserved;
+    double delta_t, mass;
+    double coupling;               /* g (Hamiltonian::coupling_a)                        */
+    double coupling_lhy;           /* g_LHY (Hamiltonian::LeeHuangYang_coupling_a)       */
+    double delta_x, delta_y;
+    double length_x, length_y;
+    double norm_target;            /* imaginary time: squared norm kept (0: none)        */
+} tsmi_desc;
+
+/* Creates a context on the current HIP device: the CPUBlock/CC2Kernel
+ * constructor (src/kernel.h:68-70, src/gpukernel.cu:122-238).  Rejects angular
+ * velocity/two components/cylindrical by construction (not in the descriptor). */
+int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out);
+
+/* Releases the context (~CPUBlock, src/cpukernel.cpp:399-411). */
+void tsmi_destroy(tsmi_ctx *ctx);
+
+/* Runs the context's work on an existing HIP stream (hipStream_t passed as a
+ * pointer; NULL returns to the context's own stream).  No reference
+ * counterpart: the reference's GPU kernel creates its own streams. */
+int tsmi_set_stream(tsmi_ctx *ctx, void *hip_stream);
+
+/* Uploads the wave function, W*H doubles each: the state buffers the kernel is
+ * constructed on (CPUBlock borrows state->p_real/p_imag, src/cpukernel.cpp:244-245). */
+int tsmi_set_state(tsmi_ctx *ctx, const double *re, const double *im);
+
+/* Uploads E = exp(-i dt V) (real time) or exp(-dt V) (imaginary time, e_im may
+ * be NULL), W*H doubles, rounded once to the context precision:
+ * ITrotterKernel::update_potential (src/trottersuzuki.h:514) and the E tables
+ * given to the constructor (src/solver.cpp:74-109,124). */
+int tsmi_set_exp_potential(tsmi_ctx *ctx, const double *e_re, const double *e_im);
+
+/* Uploads the raw potential V, W*H doubles, used by tsmi_energy only
+ * (Potential::get_value in Solver::calculate_energy_expected_values,
+ * src/solver.cpp:336). */
+int tsmi_set_potential(tsmi_ctx *ctx, const double *v);
+
+/* n_steps full steps, as the single-component loop of Solver::evolve
+ * (src/solver.cpp:186-230): per step run_kernel_on_halo + run_kernel
+ * (src/cpukernel.cpp:413-524), the periodic halo exchange except after the last
+ * step (:691-747), and in imaginary time the renormalisation of
+ * wait_for_completion (:551-570).  Asynchronous on the context stream. */
+int tsmi_evolve(tsmi_ctx *ctx, int32_t n_steps);
+
+/* Squared norm of the inner region times dx*dy:
+ * ITrotterKernel::calculate_squared_norm (src/cpukernel.cpp:526-549).  Syncs. */
+int tsmi_squared_norm(tsmi_ctx *ctx, double *out);
+
+/* Energies of the current state, as Solver::calculate_energy_expected_values
+ * computes them for one component (src/solver.cpp:244-564):
+ * out[0..5] = total, kinetic, potential, intra-species, LHY, squared norm.
+ * Needs tsmi_set_potential.  Syncs. */
+int tsmi_energy(tsmi_ctx *ctx, double out[6]);
+
+/* State::calculate_expected_values (src/model.cpp:385-536):
+ * out[0..9] = norm2, <X>, <X^2>, <Y>, <Y^2>, <P_x>, <P_x^2>, <P_y>, <P_y^2>, <L_z>.
+ * Syncs. */
+int tsmi_expected_values(tsmi_ctx *ctx, double out[10]);
+
+/* Copies a width x height window at tile position (x, y) into host arrays with
+ * row stride dest_stride: ITrotterKernel::get_sample (src/trottersuzuki.h:508,
+ * src/cpukernel.cpp:572-579), one component.  Syncs. */
+int tsmi_get_sample(tsmi_ctx *ctx, size_t dest_stride, size_t x, size_t y,
+                    size_t width, size_t height, double *dest_re, double *dest_im);
+
+/* Waits for the context's queued work (the CUDA kernel's wait_for_completion
+ * synchronisation, src/gpukernel.cu:502-521). */
+int tsmi_synchronize(tsmi_ctx *ctx);
+
+/* Tile geometry chosen for the context: out[0..5] = W, H, pitch (elements),
+ * strips, segments, rows per segment.  No reference counterpart (reporting). */
+int tsmi_geometry(const tsmi_ctx *ctx, int32_t out[6]);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *tsmi_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

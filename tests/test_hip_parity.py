@@ -1,0 +1,130 @@
+"""Parity of the HIP step (libtsmi_hip.so, through its C ABI) with the oracle.
+
+FP64: the whole tile, halo cells included, within 1e-12 relative L2 of the
+oracle after the stated number of steps, energies within 1e-12 and expected
+values within 1e-11.  FP32 bulk: within the stated FP32 tolerance.  Both sides
+get the same inputs: the same tile, the same E table (the oracle's, built with the
+C library's complex exponential) and the same imaginary-time norm target.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tsmi import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def run_hip(wl, oracle, precision=64, steps=None, tile_rows=0, calls=None):
+    from tsmi.hip import kernel_for
+    d = oracle.make_desc(wl)
+    e = oracle.exp_potential(wl.v, wl.dt, wl.imag_time)
+    k = kernel_for(wl, precision=precision, norm_target=d.norm_target if wl.imag_time else 0.0,
+                   tile_rows=tile_rows, e_planes=e)
+    try:
+        for n in (calls or [wl.steps if steps is None else steps]):
+            k.evolve(n)
+        psi = k.sample()
+        return psi, k.energy(), k.expected_values(), k.squared_norm()
+    finally:
+        k.close()
+
+
+CASES = [
+    dict(n_x=64, n_y=64, periodic_x=False, periodic_y=False, steps=1),
+    dict(n_x=64, n_y=64, periodic_x=False, periodic_y=False, steps=100, g=5.0, g_lhy=1.0),
+    dict(n_x=64, n_y=48, periodic_x=True, periodic_y=False, steps=30, g=3.0),
+    dict(n_x=48, n_y=64, periodic_x=False, periodic_y=True, steps=30, g=3.0),
+    dict(n_x=64, n_y=64, periodic_x=True, periodic_y=True, steps=50, g=2.0),
+    dict(n_x=64, n_y=64, periodic_x=False, periodic_y=False, imag_time=True, steps=100, g=10.0, g_lhy=2.0),
+    dict(n_x=64, n_y=64, periodic_x=True, periodic_y=True, imag_time=True, steps=60, g=10.0),
+    dict(n_x=601, n_y=75, periodic_x=False, periodic_y=False, steps=12, g=1.0),      # several strips, ragged
+    dict(n_x=500, n_y=90, periodic_x=True, periodic_y=True, steps=12, g=1.0),        # strips across the seam
+    dict(n_x=248, n_y=40, periodic_x=False, periodic_y=False, steps=5),              # reference column-gap width
+    dict(n_x=13, n_y=7, periodic_x=False, periodic_y=False, steps=9, g=0.5),         # tiny, odd
+    dict(n_x=8, n_y=8, periodic_x=True, periodic_y=True, steps=9, g=0.5),            # smallest periodic
+    dict(n_x=37, n_y=1, periodic_x=False, periodic_y=False, steps=9),                # single row
+    dict(n_x=40, n_y=33, periodic_x=True, periodic_y=False, imag_time=True, steps=25, g=4.0),
+]
+
+
+def case_id(c):
+    return "_".join(f"{k}{int(v) if isinstance(v, bool) else v}" for k, v in c.items())
+
+
+@pytest.mark.parametrize("case", CASES, ids=[case_id(c) for c in CASES])
+def test_fp64_matches_oracle(case, oracle):
+    wl = W.small_case(**case)
+    psi, e, x, n2 = run_hip(wl, oracle)
+    ref = oracle.evolve(wl)
+    assert rel_l2(psi, ref) < 1e-12
+    np.testing.assert_allclose(e, oracle.energy(wl, ref), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(x, oracle.expected_values(wl, ref), rtol=1e-11, atol=1e-13)
+    d = oracle.make_desc(wl)
+    re, im = np.ascontiguousarray(ref.real), np.ascontiguousarray(ref.imag)
+    assert abs(n2 - oracle.lib().ts_oracle_squared_norm(d, oracle._p(re), oracle._p(im))) < 1e-13
+
+
+@pytest.mark.parametrize("tile_rows", [2, 8, 30])
+def test_segment_heights(tile_rows, oracle):
+    """Many row segments per strip: the 4-row aprons between segments."""
+    wl = W.small_case(300, 70, True, False, steps=7, g=2.0)
+    psi, *_ = run_hip(wl, oracle, tile_rows=tile_rows)
+    assert rel_l2(psi, oracle.evolve(wl)) < 1e-12
+
+
+def test_split_evolve_calls(oracle):
+    """evolve(7) then evolve(5): the second call starts from the halo cells the
+    first left unrefreshed, as the reference kernel does across calls."""
+    wl = W.small_case(64, 64, True, True, steps=12, g=2.0)
+    psi, *_ = run_hip(wl, oracle, calls=[7, 5])
+    ref = oracle.evolve(wl, steps=7)
+    ref = oracle.evolve(wl, steps=5, psi=ref)
+    assert rel_l2(psi, ref) < 1e-12
+
+
+def test_zero_steps_is_identity(oracle):
+    wl = W.small_case(40, 40, True, True, steps=0)
+    psi, *_ = run_hip(wl, oracle, steps=0)
+    assert np.array_equal(psi, wl.psi)
+
+
+@pytest.mark.parametrize("name", ["config0", "small64_periodic_50", "small64_periodic_imag_60",
+                                  "blocks260x136_closed_20", "small64x48_periodic_x_30"])
+def test_fp64_matches_reference_golden(name, oracle):
+    """Directly against the reference's own outputs (tests/golden)."""
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    spec = json.loads(str(z["spec"]))
+    wl = getattr(W, spec["builder"])(**spec["kwargs"])
+    psi, e, x, _ = run_hip(wl, oracle, steps=int(z["steps"]))
+    ref = z["psi_re"] + 1j * z["psi_im"]
+    assert rel_l2(psi, ref) < 1e-12
+    np.testing.assert_allclose(e, z["energies"], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(x, z["expect"], rtol=1e-11, atol=1e-13)
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[4], CASES[5], CASES[8]], ids=lambda c: case_id(c))
+def test_fp32_bulk_within_tolerance(case, oracle):
+    """FP32 bulk with FP64 coefficients rounded once and FP64 reductions:
+    relative L2 below 2e-5 after up to 100 steps on these cases."""
+    wl = W.small_case(**case)
+    psi, e, x, n2 = run_hip(wl, oracle, precision=32)
+    ref = oracle.evolve(wl)
+    assert rel_l2(psi, ref) < 2e-5
+    np.testing.assert_allclose(e, oracle.energy(wl, ref), rtol=1e-5, atol=1e-7)
+    if not wl.imag_time:
+        assert abs(n2 - 1.0) < 1e-5
+
+
+def test_real_time_norm_conserved_fp64(oracle):
+    wl = W.config0(n=128, steps=200)
+    psi, e, x, n2 = run_hip(wl, oracle)
+    assert abs(n2 - 1.0) < 1e-12

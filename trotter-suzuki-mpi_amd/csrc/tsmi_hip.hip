@@ -1,0 +1,449 @@
+// tsmi_hip.hip -- C ABI of the gfx950 Trotter-Suzuki step (include/tsmi_hip.h).
+//
+// A context owns two device copies of the tile (ping-pong: a step reads one and
+// writes the other, as CPUBlock does with its two buffers, src/cpukernel.cpp:244-247),
+// the E planes, the raw potential and the coordinate tables, all in the context
+// precision except the FP64 potential/coordinates used by the reductions.
+// Rows are padded to a multiple of 64 elements; the padding is zero and never
+// written, so lanes that read past the tile edge see finite values.
+#include "tsmi_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "observables.hpp"
+#include "step_kernel.hpp"
+
+using namespace tsmi;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define TSMI_HIP(call)                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return fail(TSMI_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kBlock = 256;          // 4 wavefronts per workgroup
+constexpr int kObsBlocksMax = 1024;
+
+}  // namespace
+
+struct tsmi_ctx {
+    tsmi_desc d;
+    int W = 0, H = 0, pitch = 0, esz = 0, vec = 0;
+    int n_strips = 0, n_segs = 0, seg_rows = 0, n_waves = 0;
+    void *psi_re[2] = {nullptr, nullptr}, *psi_im[2] = {nullptr, nullptr};
+    void *e_re = nullptr, *e_im = nullptr;
+    double *v = nullptr, *xc = nullptr, *yc = nullptr;
+    double *div = nullptr;            // two divisor slots (imaginary time)
+    double *partials = nullptr;       // one per wave
+    unsigned *counter = nullptr;
+    double *obs_partials = nullptr;
+    int obs_blocks = 0;
+    int cur = 0, div_slot = 0;
+    bool div_pending = false, has_state = false, has_e = false, has_v = false;
+    hipStream_t own = nullptr, stream = nullptr;
+    double aH = 0, bH = 0, aV = 0, bV = 0, c_nl = 0, c_lhy = 0;
+    int ix0 = 0, ix1 = 0, iy0 = 0, iy1 = 0;
+};
+
+namespace {
+
+size_t plane_bytes(const tsmi_ctx *c) { return (size_t)c->pitch * (size_t)c->H * (size_t)c->esz; }
+
+// map_lattice_to_coordinate_space for one axis (src/common.cpp:45-65, Cartesian)
+void map_axis(int start, double delta, int n, double length, int count, std::vector<double> &out) {
+    out.resize(count);
+    const double cc = n * delta * 0.5;
+    for (int i = 0; i < count; ++i) {
+        double idx = start * delta + 0.5 * delta + i * delta;
+        if (idx - cc < -length * 0.5) idx += length;
+        if (idx - cc > length * 0.5) idx -= length;
+        out[i] = idx - cc;
+    }
+}
+
+int free_all(tsmi_ctx *c) {
+    for (int b = 0; b < 2; ++b) {
+        if (c->psi_re[b]) (void)hipFree(c->psi_re[b]);
+        if (c->psi_im[b]) (void)hipFree(c->psi_im[b]);
+    }
+    void *ptrs[] = {c->e_re, c->e_im, c->v, c->xc, c->yc, c->div, c->partials, c->counter, c->obs_partials};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->own) (void)hipStreamDestroy(c->own);
+    return TSMI_OK;
+}
+
+// Uploads W x H doubles into a pitched plane of the context precision.
+int upload_plane(tsmi_ctx *c, void *dst, const double *src) {
+    const size_t n = (size_t)c->W * (size_t)c->H;
+    if (c->esz == 8) {
+        TSMI_HIP(hipMemcpy2DAsync(dst, (size_t)c->pitch * 8, src, (size_t)c->W * 8, (size_t)c->W * 8, c->H,
+                                  hipMemcpyHostToDevice, c->stream));
+        TSMI_HIP(hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];   // rounded once
+        TSMI_HIP(hipMemcpy2DAsync(dst, (size_t)c->pitch * 4, tmp.data(), (size_t)c->W * 4, (size_t)c->W * 4, c->H,
+                                  hipMemcpyHostToDevice, c->stream));
+        TSMI_HIP(hipStreamSynchronize(c->stream));
+    }
+    return TSMI_OK;
+}
+
+int current_div(tsmi_ctx *c, double *out) {
+    *out = 1.0;
+    if (c->d.imag_time && c->div_pending) {
+        TSMI_HIP(hipMemcpyAsync(out, c->div + c->div_slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        TSMI_HIP(hipStreamSynchronize(c->stream));
+    }
+    return TSMI_OK;
+}
+
+template <typename T, bool IMAG, int VEC>
+int launch_step(tsmi_ctx *c, int exchange) {
+    StepParams<T> p;
+    std::memset(&p, 0, sizeof(p));
+    const int nb = 1 - c->cur;
+    p.in_re = (const T *)c->psi_re[c->cur];
+    p.in_im = (const T *)c->psi_im[c->cur];
+    p.out_re = (T *)c->psi_re[nb];
+    p.out_im = (T *)c->psi_im[nb];
+    p.e_re = (const T *)c->e_re;
+    p.e_im = (const T *)c->e_im;
+    p.W = c->W; p.H = c->H; p.pitch = c->pitch;
+    p.px = c->d.periodic_x; p.py = c->d.periodic_y; p.hx = c->d.halo_x; p.hy = c->d.halo_y;
+    p.exchange = exchange;
+    p.n_strips = c->n_strips; p.n_waves = c->n_waves; p.seg_rows = c->seg_rows;
+    p.aH = (T)c->aH; p.bH = (T)c->bH; p.aV = (T)c->aV; p.bV = (T)c->bV;
+    p.c_nl = (T)c->c_nl; p.c_lhy = (T)c->c_lhy;
+    p.use_lhy = c->c_lhy != 0.0;
+    p.ix0 = c->ix0; p.ix1 = c->ix1; p.iy0 = c->iy0; p.iy1 = c->iy1;
+    if (IMAG) {
+        p.div_in = c->div_pending ? c->div + c->div_slot : nullptr;
+        p.div_out = c->div + (1 - c->div_slot);
+        p.partials = c->partials;
+        p.counter = c->counter;
+        p.norm_target = c->d.norm_target;
+        p.dxdy = c->d.delta_x * c->d.delta_y;
+    }
+    // the grid covers every (strip, segment) pair exactly once
+    const int blocks = (c->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL((step_kernel<T, IMAG, VEC>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
+    TSMI_HIP(hipGetLastError());
+    c->cur = nb;
+    if (IMAG) {
+        c->div_slot = 1 - c->div_slot;
+        c->div_pending = true;
+    }
+    return TSMI_OK;
+}
+
+int step_once(tsmi_ctx *c, int exchange) {
+    if (c->esz == 4)
+        return c->d.imag_time ? launch_step<float, true, 4>(c, exchange) : launch_step<float, false, 4>(c, exchange);
+    return c->d.imag_time ? launch_step<double, true, 2>(c, exchange) : launch_step<double, false, 2>(c, exchange);
+}
+
+template <typename T>
+int run_observables(tsmi_ctx *c, bool energy_margin, double out[OBS_COUNT]) {
+    double div = 1.0;
+    int st = current_div(c, &div);
+    if (st) return st;
+    ObsParams<T> p;
+    std::memset(&p, 0, sizeof(p));
+    p.re = (const T *)c->psi_re[c->cur];
+    p.im = (const T *)c->psi_im[c->cur];
+    p.v = c->has_v ? c->v : nullptr;
+    p.xc = c->xc; p.yc = c->yc;
+    p.W = c->W; p.H = c->H; p.pitch = c->pitch;
+    p.x0 = c->ix0; p.x1 = c->ix1; p.y0 = c->iy0; p.y1 = c->iy1;
+    // 2-cell margin next to closed edges (src/solver.cpp:356-359, src/model.cpp:424-427)
+    p.kx0 = c->ix0 + (c->d.periodic_x ? 0 : 2);
+    p.kx1 = c->ix1 - (c->d.periodic_x ? 0 : 2);
+    p.ky0 = c->iy0 + (c->d.periodic_y ? 0 : 2);
+    p.ky1 = c->iy1 - (c->d.periodic_y ? 0 : 2);
+    p.rows_free = energy_margin && c->H == 1;
+    p.div = div;
+    p.dx = c->d.delta_x; p.dy = c->d.delta_y;
+    p.partials = c->obs_partials;
+    const int blocks = std::max(1, std::min(c->obs_blocks, c->iy1 - c->iy0));
+    hipLaunchKernelGGL((observables_kernel<T>), dim3(blocks), dim3(kObsThreads), 0, c->stream, p);
+    TSMI_HIP(hipGetLastError());
+    std::vector<double> h((size_t)blocks * OBS_COUNT);
+    TSMI_HIP(hipMemcpyAsync(h.data(), c->obs_partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TSMI_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < OBS_COUNT; ++i) {
+        double s = 0.0, comp = 0.0;
+        for (int b = 0; b < blocks; ++b) {
+            const double v = h[(size_t)b * OBS_COUNT + i];
+            const double t = s + v;
+            comp += (std::fabs(s) >= std::fabs(v)) ? (s - t) + v : (v - t) + s;
+            s = t;
+        }
+        out[i] = s + comp;
+    }
+    return TSMI_OK;
+}
+
+int observables(tsmi_ctx *c, bool energy_margin, double out[OBS_COUNT]) {
+    return c->esz == 4 ? run_observables<float>(c, energy_margin, out) : run_observables<double>(c, energy_margin, out);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tsmi_last_error(void) { return g_err.c_str(); }
+
+int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
+    if (!desc || !out) return fail(TSMI_E_INVALID, "tsmi_create: null argument");
+    *out = nullptr;
+    const tsmi_desc &d = *desc;
+    if (d.dim_x < 1 || d.dim_y < 1) return fail(TSMI_E_INVALID, "tsmi_create: lattice dimensions must be positive");
+    if (d.precision != 32 && d.precision != 64) return fail(TSMI_E_INVALID, "tsmi_create: precision must be 32 or 64");
+    if (d.halo_x != 4 || d.halo_y != 4)
+        return fail(TSMI_E_UNSUPPORTED, "tsmi_create: only the halo width 4 of a non-rotating lattice is supported");
+    if (!(d.delta_x > 0) || !(d.delta_y > 0) || !(d.mass != 0))
+        return fail(TSMI_E_INVALID, "tsmi_create: lattice spacing and mass must be non-zero");
+    if ((d.periodic_x && d.dim_x < 2 * d.halo_x) || (d.periodic_y && d.dim_y < 2 * d.halo_y))
+        return fail(TSMI_E_INVALID, "tsmi_create: a periodic axis needs at least 2*halo points");
+    const long long W = (long long)d.dim_x + (d.periodic_x ? 2 * d.halo_x : 0);
+    const long long H = (long long)d.dim_y + (d.periodic_y ? 2 * d.halo_y : 0);
+    const long long pitch = (W + 63) / 64 * 64;
+    if (pitch * H >= (1LL << 40) || W > (1 << 30) || H > (1 << 30))
+        return fail(TSMI_E_INVALID, "tsmi_create: lattice too large");
+
+    tsmi_ctx *c = new tsmi_ctx();
+    c->d = d;
+    c->W = (int)W; c->H = (int)H; c->pitch = (int)pitch;
+    c->esz = d.precision == 32 ? 4 : 8;
+    c->vec = c->esz == 4 ? 4 : 2;
+    const int sw = kWave * c->vec - 2 * kApron;
+    c->n_strips = (c->W + sw - 1) / sw;
+    int seg = d.tile_rows;
+    if (seg <= 0) {
+        const long long target = 4096;   // wavefronts per launch: several per SIMD
+        seg = (int)std::max<long long>(16, (H * c->n_strips + target - 1) / target);
+        seg = (seg + 7) / 8 * 8;
+    }
+    seg = (seg + 1) / 2 * 2;   // segments start on even rows (the kernel's pair patterns assume it)
+    c->seg_rows = std::min<int>(seg, c->H);
+    c->n_segs = (c->H + c->seg_rows - 1) / c->seg_rows;
+    c->n_waves = c->n_strips * c->n_segs;
+    c->ix0 = d.periodic_x ? d.halo_x : 0; c->ix1 = c->W - c->ix0;
+    c->iy0 = d.periodic_y ? d.halo_y : 0; c->iy1 = c->H - c->iy0;
+
+    // kinetic and nonlinear coefficients in FP64 (src/cpukernel.cpp:195-216)
+    const double bx = d.delta_t / (4. * d.mass * d.delta_x * d.delta_x);
+    const double by = d.delta_t / (4. * d.mass * d.delta_y * d.delta_y);
+    if (d.imag_time) {
+        c->aH = std::cosh(bx); c->bH = std::sinh(bx);
+        c->aV = std::cosh(by); c->bV = std::sinh(by);
+    } else {
+        c->aH = std::cos(bx); c->bH = std::sin(bx);
+        c->aV = std::cos(by); c->bV = std::sin(by);
+    }
+    c->c_nl = d.coupling * d.delta_t;
+    c->c_lhy = d.coupling_lhy * d.delta_t;
+
+    auto bail = [&](int code, const std::string &m) {
+        free_all(c);
+        delete c;
+        return fail(code, m);
+    };
+    hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(TSMI_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    c->stream = c->own;
+    const size_t pb = plane_bytes(c);
+    const size_t pd = (size_t)c->pitch * (size_t)c->H * sizeof(double);
+    c->obs_blocks = std::min(kObsBlocksMax, c->H);
+    struct { void **p; size_t n; } allocs[] = {
+        {&c->psi_re[0], pb}, {&c->psi_im[0], pb}, {&c->psi_re[1], pb}, {&c->psi_im[1], pb},
+        {&c->e_re, pb}, {&c->e_im, pb}, {(void **)&c->v, pd},
+        {(void **)&c->xc, (size_t)c->W * sizeof(double)}, {(void **)&c->yc, (size_t)c->H * sizeof(double)},
+        {(void **)&c->div, 2 * sizeof(double)}, {(void **)&c->partials, (size_t)c->n_waves * sizeof(double)},
+        {(void **)&c->counter, 64}, {(void **)&c->obs_partials, (size_t)c->obs_blocks * OBS_COUNT * sizeof(double)},
+    };
+    for (auto &a : allocs) {
+        e = hipMalloc(a.p, a.n);
+        if (e != hipSuccess) return bail(TSMI_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        e = hipMemsetAsync(*a.p, 0, a.n, c->stream);
+        if (e != hipSuccess) return bail(TSMI_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    }
+    std::vector<double> xs, ys;
+    map_axis(d.periodic_x ? -d.halo_x : 0, d.delta_x, d.dim_x, d.length_x, c->W, xs);
+    map_axis(d.periodic_y ? -d.halo_y : 0, d.delta_y, d.dim_y, d.length_y, c->H, ys);
+    e = hipMemcpyAsync(c->xc, xs.data(), xs.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->yc, ys.data(), ys.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return bail(TSMI_E_HIP, std::string("coordinate upload: ") + hipGetErrorString(e));
+    *out = c;
+    return TSMI_OK;
+}
+
+void tsmi_destroy(tsmi_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+int tsmi_set_stream(tsmi_ctx *c, void *s) {
+    if (!c) return fail(TSMI_E_INVALID, "tsmi_set_stream: null context");
+    TSMI_HIP(hipStreamSynchronize(c->stream));
+    c->stream = s ? (hipStream_t)s : c->own;
+    return TSMI_OK;
+}
+
+int tsmi_set_state(tsmi_ctx *c, const double *re, const double *im) {
+    if (!c || !re || !im) return fail(TSMI_E_INVALID, "tsmi_set_state: null argument");
+    int st = upload_plane(c, c->psi_re[c->cur], re);
+    if (!st) st = upload_plane(c, c->psi_im[c->cur], im);
+    if (st) return st;
+    c->has_state = true;
+    c->div_pending = false;
+    return TSMI_OK;
+}
+
+int tsmi_set_exp_potential(tsmi_ctx *c, const double *e_re, const double *e_im) {
+    if (!c || !e_re) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential: null argument");
+    if (!c->d.imag_time && !e_im) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential: real time needs Im(E)");
+    int st = upload_plane(c, c->e_re, e_re);
+    if (!st && e_im) st = upload_plane(c, c->e_im, e_im);
+    if (st) return st;
+    c->has_e = true;
+    return TSMI_OK;
+}
+
+int tsmi_set_potential(tsmi_ctx *c, const double *v) {
+    if (!c || !v) return fail(TSMI_E_INVALID, "tsmi_set_potential: null argument");
+    TSMI_HIP(hipMemcpy2DAsync(c->v, (size_t)c->pitch * 8, v, (size_t)c->W * 8, (size_t)c->W * 8, c->H,
+                              hipMemcpyHostToDevice, c->stream));
+    TSMI_HIP(hipStreamSynchronize(c->stream));
+    c->has_v = true;
+    return TSMI_OK;
+}
+
+int tsmi_evolve(tsmi_ctx *c, int32_t n) {
+    if (!c) return fail(TSMI_E_INVALID, "tsmi_evolve: null context");
+    if (n < 0) return fail(TSMI_E_INVALID, "tsmi_evolve: negative step count");
+    if (!c->has_state || !c->has_e) return fail(TSMI_E_INVALID, "tsmi_evolve: state and exp potential must be set");
+    for (int i = 0; i < n; ++i) {
+        const int st = step_once(c, i != n - 1 ? 1 : 0);
+        if (st) return st;
+    }
+    return TSMI_OK;
+}
+
+int tsmi_squared_norm(tsmi_ctx *c, double *out) {
+    if (!c || !out) return fail(TSMI_E_INVALID, "tsmi_squared_norm: null argument");
+    if (!c->has_state) return fail(TSMI_E_INVALID, "tsmi_squared_norm: no state");
+    double o[OBS_COUNT];
+    const int st = observables(c, true, o);
+    if (st) return st;
+    *out = o[OBS_N] * c->d.delta_x * c->d.delta_y;
+    return TSMI_OK;
+}
+
+int tsmi_energy(tsmi_ctx *c, double out[6]) {
+    if (!c || !out) return fail(TSMI_E_INVALID, "tsmi_energy: null argument");
+    if (!c->has_state || !c->has_v) return fail(TSMI_E_INVALID, "tsmi_energy: state and potential must be set");
+    double o[OBS_COUNT];
+    const int st = observables(c, true, o);
+    if (st) return st;
+    const tsmi_desc &d = c->d;
+    const double cost = -1. / (2. * d.mass);
+    const double kin = cost * (o[OBS_LAPX] / (d.delta_x * d.delta_x) +
+                               (c->H > 1 ? o[OBS_LAPY] / (d.delta_y * d.delta_y) : 0.0)) / o[OBS_NK];
+    const double pot = o[OBS_POT] / o[OBS_N];
+    const double itr = 0.5 * d.coupling * o[OBS_INT] / o[OBS_N];
+    const double lhy = 0.4 * d.coupling_lhy * o[OBS_LHY] / o[OBS_N];
+    out[0] = kin + pot + itr + lhy;                                // src/solver.cpp:538
+    out[1] = kin; out[2] = pot; out[3] = itr; out[4] = lhy;
+    out[5] = o[OBS_N] * d.delta_y * d.length_x / (double)d.dim_x;  // src/solver.cpp:561
+    return TSMI_OK;
+}
+
+int tsmi_expected_values(tsmi_ctx *c, double out[10]) {
+    if (!c || !out) return fail(TSMI_E_INVALID, "tsmi_expected_values: null argument");
+    if (!c->has_state) return fail(TSMI_E_INVALID, "tsmi_expected_values: no state");
+    double o[OBS_COUNT];
+    const int st = observables(c, false, o);
+    if (st) return st;
+    const double n = o[OBS_N], dx = c->d.delta_x, dy = c->d.delta_y;
+    const double ppx = -1. / dx, ppy = 1. / dy;                   // src/model.cpp:398-399
+    out[0] = n * (dx * dy);
+    out[1] = o[OBS_X] / n; out[2] = o[OBS_XX] / n;
+    out[3] = o[OBS_Y] / n; out[4] = o[OBS_YY] / n;
+    out[5] = -o[OBS_PX] * ppx / n;
+    out[6] = -o[OBS_LAPX] * ppx * ppx / n;
+    out[7] = -o[OBS_PY] * ppy / n;
+    out[8] = -o[OBS_LAPY] * ppy * ppy / n;
+    out[9] = o[OBS_LZ] / n;
+    return TSMI_OK;
+}
+
+int tsmi_get_sample(tsmi_ctx *c, size_t stride, size_t x, size_t y, size_t w, size_t h, double *re, double *im) {
+    if (!c || !re || !im) return fail(TSMI_E_INVALID, "tsmi_get_sample: null argument");
+    if (!c->has_state) return fail(TSMI_E_INVALID, "tsmi_get_sample: no state");
+    if (x + w > (size_t)c->W || y + h > (size_t)c->H || stride < w)
+        return fail(TSMI_E_INVALID, "tsmi_get_sample: window outside the tile");
+    if (w == 0 || h == 0) return TSMI_OK;
+    double div = 1.0;
+    int st = current_div(c, &div);
+    if (st) return st;
+    const size_t es = (size_t)c->esz;
+    std::vector<unsigned char> tr(w * h * es), ti(w * h * es);
+    const size_t off = (y * (size_t)c->pitch + x) * es;
+    TSMI_HIP(hipMemcpy2DAsync(tr.data(), w * es, (const char *)c->psi_re[c->cur] + off, (size_t)c->pitch * es, w * es, h,
+                              hipMemcpyDeviceToHost, c->stream));
+    TSMI_HIP(hipMemcpy2DAsync(ti.data(), w * es, (const char *)c->psi_im[c->cur] + off, (size_t)c->pitch * es, w * es, h,
+                              hipMemcpyDeviceToHost, c->stream));
+    TSMI_HIP(hipStreamSynchronize(c->stream));
+    const bool scale = c->d.imag_time && c->div_pending;
+    for (size_t j = 0; j < h; ++j)
+        for (size_t i = 0; i < w; ++i) {
+            double vr, vi;
+            if (es == 8) {
+                vr = ((const double *)tr.data())[j * w + i];
+                vi = ((const double *)ti.data())[j * w + i];
+            } else {
+                vr = ((const float *)tr.data())[j * w + i];
+                vi = ((const float *)ti.data())[j * w + i];
+            }
+            if (scale) { vr /= div; vi /= div; }   // src/cpukernel.cpp:559-560
+            re[j * stride + i] = vr;
+            im[j * stride + i] = vi;
+        }
+    return TSMI_OK;
+}
+
+int tsmi_synchronize(tsmi_ctx *c) {
+    if (!c) return fail(TSMI_E_INVALID, "tsmi_synchronize: null context");
+    TSMI_HIP(hipStreamSynchronize(c->stream));
+    return TSMI_OK;
+}
+
+int tsmi_geometry(const tsmi_ctx *c, int32_t out[6]) {
+    if (!c || !out) return fail(TSMI_E_INVALID, "tsmi_geometry: null argument");
+    out[0] = c->W; out[1] = c->H; out[2] = c->pitch;
+    out[3] = c->n_strips; out[4] = c->n_segs; out[5] = c->seg_rows;
+    return TSMI_OK;
+}
+
+}  // extern "C"
