@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Trotter-Suzuki step throughput on one MI355X.
+
+Workload: the 8192 x 8192 real-time GPE vortex lattice (periodic, side 64,
+g = 50, harmonic trap of weight 0.01, dt = 5e-4, 16 seeded vortices), built on
+the device.  One step = one full second-order Trotter-Suzuki step of the whole
+lattice = one launch of the fused step kernel.  The FP32-bulk and the FP64 forms
+are timed on the same GPU from the same initial state; the FP32 result after K
+steps is compared with the FP64 one (relative L2 over the inner lattice).
+
+Prints one JSON line: value = site-updates/s of the FP32 step (inner sites x
+steps / time), with the roofline of the step kernel, the FP64 figures, and the
+reference CPU kernel timed on this host's cores on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "trotter-suzuki-mpi_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+BYTES_PER_SU = {32: 24, 64: 48}   # psi read + psi write + E read per inner site (real time)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_8192_traffic.json")
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=8192, help="lattice side (default: the 8192 throughput config)")
+    ap.add_argument("--tile-rows", type=int, default=0, help="rows per wavefront segment (0: library choice)")
+    ap.add_argument("--precision", choices=["both", "32", "64"], default="both")
+    ap.add_argument("--cpu-steps", type=int, default=12, help="timed steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (for counter calibration) and exit")
+    return ap.parse_args()
+
+
+def calibrate_copy():
+    import torch
+    n = 1 << 28   # 2^28 float32 = 1 GiB read + 1 GiB written per launch
+    a = torch.rand(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel_bytes_read": 4 * n, "copy_kernel_bytes_written": 4 * n, "launches": 5}))
+
+
+def time_precision(prec, lat, psi_re, psi_im, e_re, e_im, v, params, steps, warmup, tile_rows):
+    import torch
+    from tsmi.hip import HipKernel
+    k = HipKernel(lat, delta_t=params["dt"], mass=params["mass"], coupling=params["g"], precision=prec,
+                  tile_rows=tile_rows)
+    stream = torch.cuda.current_stream()
+    k.set_stream(stream.cuda_stream)
+    k.set_exp_potential_device(e_re.data_ptr(), e_im.data_ptr())
+    k.set_potential_device(v.data_ptr())
+    k.set_state_device(psi_re.data_ptr(), psi_im.data_ptr())
+    if warmup:
+        k.evolve(warmup)
+    k.set_state_device(psi_re.data_ptr(), psi_im.data_ptr())
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    k.evolve(steps)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1)
+    out_re, out_im = torch.empty_like(psi_re), torch.empty_like(psi_im)
+    k.get_state_device(out_re.data_ptr(), out_im.data_ptr())
+    torch.cuda.synchronize()
+    norm = k.squared_norm()
+    geo = k.geometry()
+    k.close()
+    return ms, out_re, out_im, norm, geo
+
+
+def cpu_baseline(lat, psi, v, params, steps):
+    """The reference CPU kernel (oracle/_ref) on this host, on the full lattice."""
+    from oracle import pyoracle
+    from tsmi.workloads import Workload
+    threads = min(16, len(os.sched_getaffinity(0)))
+    wl = Workload("config2", lat, psi, v, params["dt"], steps, g=params["g"], mass=params["mass"])
+    sample = f"{lat.dim_x}x{lat.dim_y} periodic GPE (the benchmark lattice), 1 warm-up + {steps} timed steps"
+    if pyoracle.ref_available("fast"):
+        r = pyoracle.time_reference(wl, steps=steps, n_warmup=1, kind="fast", threads=threads)
+        su = lat.dim_x * lat.dim_y * r["steps"] / r["seconds"]
+        return {"value": su, "unit": "site-updates/s", "cores": r["threads"], "kind": "reference",
+                "sample": "reference CPUBlock built from its sources (-Ofast -march=x86-64-v3 -fopenmp), " + sample}
+    # the oracle restatement (OpenMP over rows) when the reference build is absent
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import numpy as np
+    d = pyoracle.make_desc(wl, norm_target=1.0)
+    er, ei = pyoracle.exp_potential(v, params["dt"], False)
+    re, im = np.ascontiguousarray(psi.real), np.ascontiguousarray(psi.imag)
+    pyoracle.lib().ts_oracle_evolve(d, 1, pyoracle._p(re), pyoracle._p(im), pyoracle._p(er), pyoracle._p(ei))
+    t = time.perf_counter()
+    pyoracle.lib().ts_oracle_evolve(d, steps, pyoracle._p(re), pyoracle._p(im), pyoracle._p(er), pyoracle._p(ei))
+    s = time.perf_counter() - t
+    return {"value": lat.dim_x * lat.dim_y * steps / s, "unit": "site-updates/s", "cores": threads, "kind": "port",
+            "sample": "oracle restatement (OpenMP over rows), " + sample}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build runs the step on a single MI355X "
+              "(multi-GPU sharding is a design note, not part of this build); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    from tsmi import workloads_torch as WT
+
+    torch.cuda.set_device(0)
+    log(f"building the {args.n}^2 vortex lattice on the device")
+    lat, psi, v, params = WT.config2(n=args.n, device="cuda")
+    psi_re, psi_im = psi.real.contiguous(), psi.imag.contiguous()
+    e_re, e_im = WT.exp_potential(v, params["dt"], False)
+    sites = lat.dim_x * lat.dim_y
+
+    results = {}
+    precs = [32, 64] if args.precision == "both" else [int(args.precision)]
+    for prec in precs:
+        log(f"FP{prec}: {args.warmup} warm-up + {args.steps} timed steps")
+        ms, o_re, o_im, norm, geo = time_precision(prec, lat, psi_re, psi_im, e_re, e_im, v, params,
+                                                   args.steps, args.warmup, args.tile_rows)
+        per_step = ms / args.steps
+        su = sites * args.steps / (ms * 1e-3)
+        ach = BYTES_PER_SU[prec] * sites / (per_step * 1e-3) / 1e9
+        results[prec] = dict(ms=ms, per_step=per_step, su=su, achieved=ach, norm=norm, geo=geo, out=(o_re, o_im))
+        log(f"FP{prec}: {per_step * 1e3:.1f} us/step, {su:.4g} SU/s, {ach:.0f} GB/s algorithmic, norm {norm:.15f}")
+
+    rel = None
+    if 32 in results and 64 in results:
+        iy, ix = lat.inner_y, lat.inner_x
+        a32 = torch.complex(*[t[iy, ix] for t in results[32]["out"]])
+        a64 = torch.complex(*[t[iy, ix] for t in results[64]["out"]])
+        rel = float(torch.linalg.vector_norm(a32 - a64) / torch.linalg.vector_norm(a64))
+        log(f"FP32 vs FP64 after {args.steps} steps: relative L2 {rel:.3e}")
+
+    main_prec = precs[0]
+    r = results[main_prec]
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f).get(f"fp{main_prec}")
+        if tr and tr.get("n") == args.n:
+            traffic = tr.get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"CPU baseline: reference kernel, {args.cpu_steps} steps of the full lattice")
+        psi_h = psi.cpu().numpy()
+        v_h = v.cpu().numpy()
+        cpu = cpu_baseline(lat, psi_h, v_h, params, args.cpu_steps)
+        log(f"CPU baseline: {cpu['value']:.4g} SU/s on {cpu['cores']} threads ({cpu['kind']})")
+
+    line = {
+        "metric": "site-updates/s per Trotter step & % HBM peak; FP32 vs FP64 at fixed L2 error",
+        "value": r["su"],
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"f{main_prec}",
+        "data": "synthetic (seeded 16-vortex Thomas-Fermi state built on the device)",
+        "config": {"workload": f"real-time GPE vortex dynamics, {lat.dim_x}x{lat.dim_y} periodic, side 64, "
+                               "g=50, trap 0.01, dt=5e-4",
+                   "lattice": [lat.dim_x, lat.dim_y], "tile": [lat.W, lat.H], "geometry": r["geo"],
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_site": BYTES_PER_SU[main_prec], "sweeps_per_launch": 1},
+        "cpu_baseline": cpu,
+    }
+    if 64 in results and main_prec != 64:
+        r64 = results[64]
+        line["fp64"] = {"value": r64["su"], "ms_per_step": r64["per_step"],
+                        "roofline": {"bound": "hbm", "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
+                                     "bytes_per_site": BYTES_PER_SU[64]}}
+        line["fp32_over_fp64"] = r["su"] / r64["su"]
+        line["fp32_rel_l2_vs_fp64"] = rel
+        line["norm_drift"] = {"fp32": abs(results[32]["norm"] - 1.0), "fp64": abs(r64["norm"] - 1.0)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -63,13 +63,18 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out);
 void tsmi_destroy(tsmi_ctx *ctx);
 
 /* Runs the context's work on an existing HIP stream (hipStream_t passed as a
- * pointer; NULL returns to the context's own stream).  No reference
- * counterpart: the reference's GPU kernel creates its own streams. */
+ * pointer; NULL is the HIP null stream).  Until this is called the context uses
+ * a non-blocking stream of its own.  No reference counterpart: the reference's
+ * GPU kernel creates its own streams. */
 int tsmi_set_stream(tsmi_ctx *ctx, void *hip_stream);
 
 /* Uploads the wave function, W*H doubles each: the state buffers the kernel is
  * constructed on (CPUBlock borrows state->p_real/p_imag, src/cpukernel.cpp:244-245). */
 int tsmi_set_state(tsmi_ctx *ctx, const double *re, const double *im);
+
+/* The same, from device memory: re/im point to W*H FP64 arrays (row stride W)
+ * on the context's device.  Asynchronous on the context stream. */
+int tsmi_set_state_device(tsmi_ctx *ctx, const double *re, const double *im);
 
 /* Uploads E = exp(-i dt V) (real time) or exp(-dt V) (imaginary time, e_im may
  * be NULL), W*H doubles, rounded once to the context precision:
@@ -77,10 +82,16 @@ int tsmi_set_state(tsmi_ctx *ctx, const double *re, const double *im);
  * given to the constructor (src/solver.cpp:74-109,124). */
 int tsmi_set_exp_potential(tsmi_ctx *ctx, const double *e_re, const double *e_im);
 
+/* The same, from device memory (W*H FP64, row stride W).  Asynchronous. */
+int tsmi_set_exp_potential_device(tsmi_ctx *ctx, const double *e_re, const double *e_im);
+
 /* Uploads the raw potential V, W*H doubles, used by tsmi_energy only
  * (Potential::get_value in Solver::calculate_energy_expected_values,
  * src/solver.cpp:336). */
 int tsmi_set_potential(tsmi_ctx *ctx, const double *v);
+
+/* The same, from device memory (W*H FP64, row stride W).  Asynchronous. */
+int tsmi_set_potential_device(tsmi_ctx *ctx, const double *v);
 
 /* n_steps full steps, as the single-component loop of Solver::evolve
  * (src/solver.cpp:186-230): per step run_kernel_on_halo + run_kernel
@@ -109,6 +120,11 @@ int tsmi_expected_values(tsmi_ctx *ctx, double out[10]);
  * src/cpukernel.cpp:572-579), one component.  Syncs. */
 int tsmi_get_sample(tsmi_ctx *ctx, size_t dest_stride, size_t x, size_t y,
                     size_t width, size_t height, double *dest_re, double *dest_im);
+
+/* The whole tile into device memory: W*H FP64 arrays (row stride W), with a
+ * pending imaginary-time renormalisation applied (get_sample into device
+ * buffers).  Asynchronous on the context stream. */
+int tsmi_get_state_device(tsmi_ctx *ctx, double *re, double *im);
 
 /* Waits for the context's queued work (the CUDA kernel's wait_for_completion
  * synchronisation, src/gpukernel.cu:502-521). */

@@ -128,3 +128,24 @@ def test_real_time_norm_conserved_fp64(oracle):
     wl = W.config0(n=128, steps=200)
     psi, e, x, n2 = run_hip(wl, oracle)
     assert abs(n2 - 1.0) < 1e-12
+
+
+def test_fp32_high_beta_accuracy_1000_steps(oracle):
+    """FP32 bulk against FP64 on the same GPU after 1000 steps at the kinetic
+    angle beta = dt/(4 dx^2) = 2.048 of the 8192^2 benchmark lattice (512^2,
+    side 4): relative L2 below 1e-4 and norm drift below 2e-5."""
+    import torch
+    from tsmi.hip import HipKernel, exp_potential
+    wl = W.small_case(512, 512, True, True, steps=1000, dt=5e-4, g=50.0, length=4.0)
+    outs = {}
+    for prec in (32, 64):
+        k = HipKernel(wl.lattice, delta_t=wl.dt, coupling=wl.g, precision=prec)
+        k.set_exp_potential(*exp_potential(wl.v, wl.dt, False))
+        k.set_state(wl.psi)
+        k.evolve(wl.steps)
+        outs[prec] = (k.sample(), k.squared_norm())
+        k.close()
+    lat = wl.lattice
+    a, b = outs[32][0][lat.inner_y, lat.inner_x], outs[64][0][lat.inner_y, lat.inner_x]
+    assert rel_l2(a, b) < 1e-4
+    assert abs(outs[32][1] - outs[64][1]) < 2e-5

@@ -19,12 +19,11 @@
 // before the multiplication by E (imaginary time: psi <- exp(-(c n + c_lhy n^1.5)) Re(E) psi).
 //
 // The tile is the reference tile W x H (halo included on periodic axes).  The
-// sweep is closed at the tile edges; for a periodic axis the halo cells carry the
-// wrapped copies written by the previous step.  In "exchange" mode a step writes
-// those copies itself (the single-process halo exchange, src/cpukernel.cpp:707-715
-// then :738-746, including its row range H - 2*halo_y for the x halo); in the last
-// step of an evolve call it writes the computed halo cells instead, as the
-// reference does when it skips the exchange (src/solver.cpp:203-209).
+// sweep is closed at the tile edges and writes every cell; on a periodic axis the
+// halo cells then receive the wrapped copies from halo_exchange_kernel (the
+// single-process halo exchange, src/cpukernel.cpp:707-715 then :738-746), which
+// the host skips after the last step of an evolve call as the reference does
+// (src/solver.cpp:203-209).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -48,11 +47,8 @@ struct StepParams {
     T *out_re, *out_im;
     const T *e_re, *e_im;        // E = exp(-i dt V) planes (e_im unused in imaginary time)
     int W, H, pitch;             // tile and row pitch (elements)
-    int px, py, hx, hy;          // periodic flags and lattice halo widths
-    int exchange;                // 1: write periodic halo copies, 0: write computed halo cells
     int n_strips, n_waves, seg_rows;
     T aH, bH, aV, bV, c_nl, c_lhy;
-    int use_lhy;
     // imaginary time only
     const double *div_in;        // divisor of the previous renormalisation (nullptr: 1)
     double *div_out;             // divisor computed from this step's output
@@ -164,33 +160,140 @@ __device__ __forceinline__ void vpair(Row<T, VEC> &u, Row<T, VEC> &l, T a, T b) 
     }
 }
 
-__device__ __forceinline__ void sincos_t(float x, float *s, float *c) { sincosf(x, s, c); }
+// sin and cos of a float angle: three-part Cody-Waite reduction by pi/2 with
+// fused multiply-adds (the reduced argument stays accurate to |x| ~ 2^22, far
+// beyond any phase a Trotter step applies), then the Taylor polynomials of sin
+// (to r^9) and cos (to r^10) on |r| <= pi/4, whose truncation errors (2e-9,
+// 1e-10) are below half an ulp; the quadrant comes from the reduction integer.
+__device__ __forceinline__ void sincos_t(float x, float *s, float *c) {
+    {
+        const float k = rintf(x * 0.636619772367581343f);
+        float r = fmaf(-k, 1.57079637050628662109375f, x);
+        r = fmaf(-k, -4.37113900018624283e-8f, r);
+        r = fmaf(-k, -1.71512451334011e-15f, r);
+        const float r2 = r * r;
+        float ps = fmaf(r2, 2.7557319e-6f, -1.98412698e-4f);
+        ps = fmaf(r2, ps, 8.33333333e-3f);
+        ps = fmaf(r2, ps, -1.66666667e-1f);
+        const float sn = fmaf(r * r2, ps, r);
+        float pc = fmaf(r2, -2.7557319e-7f, 2.48015873e-5f);
+        pc = fmaf(r2, pc, -1.38888889e-3f);
+        pc = fmaf(r2, pc, 4.16666667e-2f);
+        pc = fmaf(r2, pc, -0.5f);
+        const float cs = fmaf(r2, pc, 1.0f);
+        const int q = (int)k;
+        const float a = (q & 1) ? cs : sn;
+        const float b = (q & 1) ? sn : cs;
+        *s = (q & 2) ? -a : a;
+        *c = ((q + 1) & 2) ? -b : b;
+    }
+}
 __device__ __forceinline__ void sincos_t(double x, double *s, double *c) { sincos(x, s, c); }
 __device__ __forceinline__ float exp_t(float x) { return expf(x); }
 __device__ __forceinline__ double exp_t(double x) { return exp(x); }
 __device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 __device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
 
+// True when no lane of the wave holds a value above thr.
+__device__ __forceinline__ bool wave_all_le(float v, float thr) {
+    return __builtin_amdgcn_ballot_w64(!(v <= thr)) == 0;
+}
+__device__ __forceinline__ bool wave_all_le(double v, double thr) {
+    return __builtin_amdgcn_ballot_w64(!(v <= thr)) == 0;
+}
+
+// sin/cos and exp(-x) of the nonlinear phase.  The phase g*dt*|psi|^2 of one
+// step is small for any usable dt, so when every lane's |x| <= 1/64 (a
+// wave-uniform test) short Taylor polynomials are already correctly rounded
+// to within the format: truncation below 2e-10 relative in FP32 (sin to x^3,
+// cos to x^4, exp to x^4) and below 1e-19 in FP64 (sin to x^7, cos to x^8,
+// exp to x^8).  Otherwise the general routines run.
+template <typename T, int VEC>
+__device__ __forceinline__ void sincos_row(const T (&x)[VEC], T (&s)[VEC], T (&c)[VEC]) {
+    T m = fabs(x[0]);
+#pragma unroll
+    for (int k = 1; k < VEC; ++k) m = fmax(m, fabs(x[k]));
+    if (wave_all_le(m, T(0.015625))) {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const T r = x[k], r2 = r * r;
+            if constexpr (sizeof(T) == 4) {
+                s[k] = fmaf(r * r2, -0.166666667f, r);
+                c[k] = fmaf(r2, fmaf(r2, 0.0416666667f, -0.5f), 1.0f);
+            } else {
+                T ps = fma(r2, -1.0 / 5040.0, 1.0 / 120.0);
+                ps = fma(r2, ps, -1.0 / 6.0);
+                s[k] = fma(r * r2, ps, r);
+                T pc = fma(r2, 1.0 / 40320.0, -1.0 / 720.0);
+                pc = fma(r2, pc, 1.0 / 24.0);
+                pc = fma(r2, pc, -0.5);
+                c[k] = fma(r2, pc, 1.0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) sincos_t(x[k], &s[k], &c[k]);
+    }
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void expneg_row(const T (&x)[VEC], T (&e)[VEC]) {
+    T m = fabs(x[0]);
+#pragma unroll
+    for (int k = 1; k < VEC; ++k) m = fmax(m, fabs(x[k]));
+    if (wave_all_le(m, T(0.015625))) {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const T r = -x[k];
+            if constexpr (sizeof(T) == 4) {
+                float q = fmaf(r, 0.0416666667f, 0.166666667f);
+                q = fmaf(r, q, 0.5f);
+                q = fmaf(r, q, 1.0f);
+                e[k] = fmaf(r, q, 1.0f);
+            } else {
+                double q = fma(r, 1.0 / 40320.0, 1.0 / 5040.0);
+                q = fma(r, q, 1.0 / 720.0);
+                q = fma(r, q, 1.0 / 120.0);
+                q = fma(r, q, 1.0 / 24.0);
+                q = fma(r, q, 1.0 / 6.0);
+                q = fma(r, q, 0.5);
+                q = fma(r, q, 1.0);
+                e[k] = fma(r, q, 1.0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) e[k] = exp_t(-x[k]);
+    }
+}
+
 // P on one row (src/cpucartesian.cpp:93-109 / :127-137).
-template <bool IMAG, typename T, int VEC>
-__device__ __forceinline__ void ppass(Row<T, VEC> &r, const Row<T, VEC> &e, T c, T cl, bool use_lhy) {
+template <bool IMAG, bool LHY, typename T, int VEC>
+__device__ __forceinline__ void ppass(Row<T, VEC> &r, const Row<T, VEC> &e, T c, T cl) {
+    T ang[VEC];
 #pragma unroll
     for (int k = 0; k < VEC; ++k) {
-        const T pr = r.re[k], pi = r.im[k];
-        const T n2 = pr * pr + pi * pi;
-        T ang = c * n2;
-        if (use_lhy) ang = c * n2 + cl * (n2 * sqrt_t(n2));
-        if (!IMAG) {
-            T sn, cs;
-            sincos_t(ang, &sn, &cs);
+        const T n2 = r.re[k] * r.re[k] + r.im[k] * r.im[k];
+        ang[k] = LHY ? c * n2 + cl * (n2 * sqrt_t(n2)) : c * n2;
+    }
+    if (!IMAG) {
+        T sn[VEC], cs[VEC];
+        sincos_row<T, VEC>(ang, sn, cs);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const T pr = r.re[k], pi = r.im[k];
             const T r1 = e.re[k] * pr - e.im[k] * pi;
             const T i1 = e.re[k] * pi + e.im[k] * pr;
-            r.re[k] = cs * r1 + sn * i1;
-            r.im[k] = cs * i1 - sn * r1;
-        } else {
-            const T f = exp_t(T(-1) * ang);
-            r.re[k] = f * e.re[k] * pr;
-            r.im[k] = f * e.re[k] * pi;
+            r.re[k] = cs[k] * r1 + sn[k] * i1;
+            r.im[k] = cs[k] * i1 - sn[k] * r1;
+        }
+    } else {
+        T f[VEC];
+        expneg_row<T, VEC>(ang, f);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            r.re[k] = f[k] * e.re[k] * r.re[k];
+            r.im[k] = f[k] * e.re[k] * r.im[k];
         }
     }
 }
@@ -223,58 +326,55 @@ __device__ __forceinline__ void store_vec(T *p, const T (&v)[VEC]) {
     *reinterpret_cast<V *>(p) = x;
 }
 
-// ---------------------------------------------------------------------------
-// halo bookkeeping for the exchange store rule
-// ---------------------------------------------------------------------------
-struct HaloMap {
-    int W, H, px, py, hx, hy, nx, ny;   // nx = W - 2hx on a periodic x axis
-    int rx0, rx1;                       // rows whose x halo the exchange refreshes
-    __device__ bool xdest(int x, int y) const { return px && (x < hx || x >= W - hx) && y >= rx0 && y < rx1; }
-    __device__ bool ydest(int y) const { return py && (y < hy || y >= H - hy); }
-    // column receiving the x copy of source column x (or -1)
-    __device__ int ximg(int x) const {
-        if (!px) return -1;
-        if (x >= hx && x < 2 * hx) return x + nx;
-        if (x >= W - 2 * hx && x < W - hx) return x - nx;
-        return -1;
-    }
-    __device__ int yimg(int y) const {
-        if (!py) return -1;
-        if (y >= hy && y < 2 * hy) return y + ny;
-        if (y >= H - 2 * hy && y < H - hy) return y - ny;
-        return -1;
-    }
-};
+// Buffer-resource access: the base is the first row of the wave's segment, the
+// lane offset is constant and the row offset is uniform, so addressing costs no
+// vector instructions, and offsets outside the resource read as zero (stores are
+// dropped) -- the lanes that must not touch memory get an offset past the end.
+typedef unsigned __attribute__((ext_vector_type(4))) u32x4;
+constexpr unsigned kOffNone = 0x80000000u;
 
-template <typename T>
-__device__ __forceinline__ void put(T *re, T *im, long off, T vr, T vi) {
-    re[off] = vr;
-    im[off] = vi;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
 }
 
-// Writes the final value of cell (x, y) and its periodic copies (exchange mode).
-template <typename T>
-__device__ __forceinline__ void store_cell_exchange(const StepParams<T> &p, const HaloMap &m, int x, int y, T vr, T vi) {
-    const long row = (long)y * p.pitch;
-    if (!m.xdest(x, y) && !m.ydest(y)) put(p.out_re, p.out_im, row + x, vr, vi);
-    const bool xsrc_row = y >= m.rx0 && y < m.rx1;
-    const int xi = xsrc_row ? m.ximg(x) : -1;
-    if (xi >= 0 && !m.ydest(y)) put(p.out_re, p.out_im, row + xi, vr, vi);
-    const int yi = m.yimg(y);
-    if (yi >= 0) {
-        const long yrow = (long)yi * p.pitch;
-        if (!m.xdest(x, y)) put(p.out_re, p.out_im, yrow + x, vr, vi);
-        if (xi >= 0) put(p.out_re, p.out_im, yrow + xi, vr, vi);
+template <typename T, int VEC>
+__device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, T (&v)[VEC]) {
+    static_assert(sizeof(T) * VEC % 16 == 0, "rows are loaded in 16-byte pieces");
+    constexpr int P = sizeof(T) * VEC / 16;
+    constexpr int E = 16 / sizeof(T);
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        const u32x4 w = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * i, soff, 0));
+        typedef T __attribute__((ext_vector_type(E))) TV;
+        const TV x = __builtin_bit_cast(TV, w);
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[i * E + k] = x[k];
+    }
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const T (&v)[VEC]) {
+    constexpr int P = sizeof(T) * VEC / 16;
+    constexpr int E = 16 / sizeof(T);
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        typedef T __attribute__((ext_vector_type(E))) TV;
+        TV x;
+#pragma unroll
+        for (int k = 0; k < E; ++k) x[k] = v[i * E + k];
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, voff + 16 * i, soff, 0);
     }
 }
 
 // ---------------------------------------------------------------------------
 // the kernel
 // ---------------------------------------------------------------------------
-template <typename T, bool IMAG, int VEC>
+template <typename T, bool IMAG, int VEC, bool LHY>
 __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    // wave-uniform by construction; readfirstlane lets the compiler keep every
+    // quantity derived from it (segment, strip, row bounds) in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
     constexpr int SW = kWave * VEC - 2 * kApron;       // output columns per strip
     const int strip = wave % p.n_strips;
@@ -297,133 +397,129 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
     double divd = 1.0;
     if (IMAG && p.div_in) divd = *p.div_in;
 
-    HaloMap hm;
-    hm.W = p.W; hm.H = p.H; hm.px = p.px; hm.py = p.py; hm.hx = p.hx; hm.hy = p.hy;
-    hm.nx = p.W - 2 * p.hx; hm.ny = p.H - 2 * p.hy;
-    hm.rx0 = p.py ? p.hy : 0;
-    hm.rx1 = hm.rx0 + max(0, p.H - 2 * p.hy);
-    const bool strip_plain = !p.exchange || !p.px ||
-                             (X0 + kApron >= 2 * p.hx && X0 + kApron + SW <= p.W - 2 * p.hx);
-
     double acc = 0.0, comp = 0.0;   // Neumaier sum of |psi|^2 over stored inner cells
 
+    // segment-relative buffer resources
+    const long seg0 = (long)y_lo * p.pitch;
+    const unsigned seg_bytes = (unsigned)((long)(y_hi - y_lo) * p.pitch * (long)sizeof(T));
+    const __amdgpu_buffer_rsrc_t r_in_re = make_rsrc(p.in_re + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_in_im = make_rsrc(p.in_im + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_e_re = make_rsrc(p.e_re + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_e_im = make_rsrc(IMAG ? p.e_re + seg0 : p.e_im + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_out_re = make_rsrc(p.out_re + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_out_im = make_rsrc(p.out_im + seg0, seg_bytes);
+    const unsigned row_bytes = (unsigned)p.pitch * (unsigned)sizeof(T);
+    const unsigned v_ld = ld_ok ? (unsigned)xl * (unsigned)sizeof(T) : kOffNone;
+    // own lanes store whole VEC-column pieces; columns in [W, pitch) are padding
+    // and are written back as the zeros they hold
+    const unsigned v_st = (own && ld_ok) ? (unsigned)xl * (unsigned)sizeof(T) : kOffNone;
+    // inner columns of this lane for the imaginary-time sum
+    unsigned inner_mask = 0;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k)
+        if (own && xl + k >= p.ix0 && xl + k < p.ix1) inner_mask |= 1u << k;
+
     auto load_psi = [&](int y, Row<T, VEC> &r) {
-        if (ld_ok) {
-            const long off = (long)y * p.pitch + xl;
-            load_vec<T, VEC>(p.in_re + off, r.re);
-            load_vec<T, VEC>(p.in_im + off, r.im);
-            if (IMAG && p.div_in) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_load<T, VEC>(r_in_re, v_ld, so, r.re);
+        buf_load<T, VEC>(r_in_im, v_ld, so, r.im);
+        if (IMAG && p.div_in) {
 #pragma unroll
-                for (int k = 0; k < VEC; ++k) {
-                    r.re[k] = T(double(r.re[k]) / divd);
-                    r.im[k] = T(double(r.im[k]) / divd);
-                }
+            for (int k = 0; k < VEC; ++k) {
+                r.re[k] = T(double(r.re[k]) / divd);
+                r.im[k] = T(double(r.im[k]) / divd);
             }
-        } else {
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) { r.re[k] = T(0); r.im[k] = T(0); }
         }
     };
     auto load_e = [&](int y, Row<T, VEC> &r) {
-        if (ld_ok && y >= y_lo && y < y_hi) {
-            const long off = (long)y * p.pitch + xl;
-            load_vec<T, VEC>(p.e_re + off, r.re);
-            if (!IMAG) load_vec<T, VEC>(p.e_im + off, r.im);
-            else {
-#pragma unroll
-                for (int k = 0; k < VEC; ++k) r.im[k] = T(0);
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) { r.re[k] = T(1); r.im[k] = T(0); }
-        }
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_load<T, VEC>(r_e_re, v_ld, so, r.re);
+        if (!IMAG) buf_load<T, VEC>(r_e_im, v_ld, so, r.im);
     };
 
-    Row<T, VEC> pend1, pend2, pend3, pend4;
-    Row<T, VEC> inA, inB, eA, eB;
+    // Every row register starts at zero: rows outside the segment stay finite,
+    // and a pair whose partner row does not exist gets the coefficients (1, 0).
+    Row<T, VEC> pend1, pend2, pend3, pend4, inA, inB, eA, eB;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        pend1.re[k] = pend1.im[k] = pend2.re[k] = pend2.im[k] = T(0);
+        pend3.re[k] = pend3.im[k] = pend4.re[k] = pend4.im[k] = T(0);
+        inA.re[k] = inA.im[k] = inB.re[k] = inB.im[k] = T(0);
+        eA.re[k] = eA.im[k] = eB.re[k] = eB.im[k] = T(0);
+    }
 
     auto valid = [&](int y) { return y >= y_lo && y < y_hi; };
 
-    // One pipeline beat for input row t; TP = t mod 2 fixes every pair pattern
-    // (rows handled by a wave start at an even row, see the host launcher).
-    auto process = [&](auto tp, int t, Row<T, VEC> &in, const Row<T, VEC> &e) {
-        constexpr int TP = decltype(tp)::value;
-        // stage 1: V0, emits row t-1
-        Row<T, VEC> o1;
-        if (valid(t) && valid(t - 1)) vpair<(TP + 1) & 1, IMAG>(pend1, in, aV, bV);
-        o1 = pend1;
-        if (valid(t)) pend1 = in;
-        if (valid(t - 1)) hpass<(TP + 1) & 1, IMAG>(o1, hc);
-        // stage 2: V1, emits row t-2
-        Row<T, VEC> o2;
-        if (valid(t - 1) && valid(t - 2)) vpair<(TP + 2 + 1) & 1, IMAG>(pend2, o1, aV, bV);
-        o2 = pend2;
-        if (valid(t - 1)) pend2 = o1;
-        if (valid(t - 2)) {
-            hpass<(TP + 2 + 1) & 1, IMAG>(o2, hc);
-            ppass<IMAG>(o2, e, p.c_nl, p.c_lhy, p.use_lhy != 0);
-            hpass<(TP + 2 + 1) & 1, IMAG>(o2, hc);
-        }
-        // stage 3: V1, emits row t-3
-        Row<T, VEC> o3;
-        if (valid(t - 2) && valid(t - 3)) vpair<(TP + 3 + 1) & 1, IMAG>(pend3, o2, aV, bV);
-        o3 = pend3;
-        if (valid(t - 2)) pend3 = o2;
-        if (valid(t - 3)) hpass<(TP + 3) & 1, IMAG>(o3, hc);
-        // stage 4: V0, emits row t-4
-        Row<T, VEC> o4;
-        if (valid(t - 3) && valid(t - 4)) vpair<(TP + 4) & 1, IMAG>(pend4, o3, aV, bV);
-        o4 = pend4;
-        if (valid(t - 3)) pend4 = o3;
-        const int y = t - 4;
-        if (y >= y_out_lo && y < y_out_hi && own) {
-            const bool row_plain = strip_plain &&
-                (!p.exchange || !p.py || (y >= 2 * p.hy && y < p.H - 2 * p.hy));
-            if (row_plain && xl + VEC <= p.W) {
-                const long off = (long)y * p.pitch + xl;
-                store_vec<T, VEC>(p.out_re + off, o4.re);
-                store_vec<T, VEC>(p.out_im + off, o4.im);
-            } else {
+    // Writes the row leaving the pipeline.
+    auto emit = [&](int y, const Row<T, VEC> &o4) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_store<T, VEC>(r_out_re, v_st, so, o4.re);
+        buf_store<T, VEC>(r_out_im, v_st, so, o4.im);
+        if (IMAG && y >= p.iy0 && y < p.iy1) {
 #pragma unroll
-                for (int k = 0; k < VEC; ++k) {
-                    const int x = xl + k;
-                    if (x < 0 || x >= p.W) continue;
-                    if (!p.exchange) put(p.out_re, p.out_im, (long)y * p.pitch + x, o4.re[k], o4.im[k]);
-                    else store_cell_exchange(p, hm, x, y, o4.re[k], o4.im[k]);
-                }
-            }
-            if (IMAG && y >= p.iy0 && y < p.iy1) {
-#pragma unroll
-                for (int k = 0; k < VEC; ++k) {
-                    const int x = xl + k;
-                    if (x >= p.ix0 && x < p.ix1) {
-                        const double vr = double(o4.re[k]), vi = double(o4.im[k]);
-                        const double v = vr * vr + vi * vi;
-                        const double t2 = acc + v;
-                        comp += (fabs(acc) >= fabs(v)) ? (acc - t2) + v : (v - t2) + acc;
-                        acc = t2;
-                    }
+            for (int k = 0; k < VEC; ++k) {
+                if (inner_mask & (1u << k)) {
+                    const double vr = double(o4.re[k]), vi = double(o4.im[k]);
+                    const double v = vr * vr + vi * vi;
+                    const double t2 = acc + v;
+                    comp += (fabs(acc) >= fabs(v)) ? (acc - t2) + v : (v - t2) + acc;
+                    acc = t2;
                 }
             }
         }
     };
 
-    // software pipeline: rows t+1 and t+2 (and E rows t-1, t) are in flight while row t runs
+    // One pipeline beat for input row t: stage k takes row t-k+1 and emits row
+    // t-k.  TP = t mod 2 fixes every pair pattern (segments start on even rows,
+    // see the host launcher).
+    auto beat = [&](auto tp, int t, Row<T, VEC> &in, const Row<T, VEC> &e) {
+        constexpr int TP = decltype(tp)::value;
+        const bool v0 = valid(t), v1 = valid(t - 1), v2 = valid(t - 2), v3 = valid(t - 3), v4 = valid(t - 4);
+        const T a1 = (v0 && v1) ? aV : T(1), b1 = (v0 && v1) ? bV : T(0);
+        const T a2 = (v1 && v2) ? aV : T(1), b2 = (v1 && v2) ? bV : T(0);
+        const T a3 = (v2 && v3) ? aV : T(1), b3 = (v2 && v3) ? bV : T(0);
+        const T a4 = (v3 && v4) ? aV : T(1), b4 = (v3 && v4) ? bV : T(0);
+        // stage 1: V0, then H0 on row t-1
+        vpair<(TP + 1) & 1, IMAG>(pend1, in, a1, b1);
+        Row<T, VEC> o1 = pend1;
+        pend1 = in;
+        hpass<(TP + 1) & 1, IMAG>(o1, hc);
+        // stage 2: V1, then H1 P H1 on row t-2
+        vpair<(TP + 1) & 1, IMAG>(pend2, o1, a2, b2);
+        Row<T, VEC> o2 = pend2;
+        pend2 = o1;
+        hpass<(TP + 1) & 1, IMAG>(o2, hc);
+        ppass<IMAG, LHY>(o2, e, p.c_nl, p.c_lhy);
+        hpass<(TP + 1) & 1, IMAG>(o2, hc);
+        // stage 3: V1, then H0 on row t-3
+        vpair<TP & 1, IMAG>(pend3, o2, a3, b3);
+        Row<T, VEC> o3 = pend3;
+        pend3 = o2;
+        hpass<(TP + 1) & 1, IMAG>(o3, hc);
+        // stage 4: V0; row t-4 leaves
+        vpair<TP & 1, IMAG>(pend4, o3, a4, b4);
+        const Row<T, VEC> o4 = pend4;
+        pend4 = o3;
+        const int y = t - 4;
+        if (v4 && y >= y_out_lo && y < y_out_hi) emit(y, o4);
+    };
+
+    // Software pipeline: rows t+1, t+2 and E rows t-1, t are in flight while
+    // row t runs.  A row that is not loaded (past the segment) keeps the finite
+    // values of an older row; every pair touching it has coefficients (1, 0).
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
     int t = y_lo;
     if (t < y_hi) load_psi(t, inA);
-    load_e(t - 2, eA);
     if (t + 1 < y_hi) load_psi(t + 1, inB);
-    load_e(t - 1, eB);
     const int t_end = y_hi + 4;
     for (; t < t_end; t += 2) {
-        process(std::integral_constant<int, 0>(), t, inA, eA);
+        beat(I0(), t, inA, eA);
         if (t + 2 < y_hi) load_psi(t + 2, inA);
-        load_e(t, eA);
-        if (t + 1 < t_end) {
-            process(std::integral_constant<int, 1>(), t + 1, inB, eB);
-            if (t + 3 < y_hi) load_psi(t + 3, inB);
-            load_e(t + 1, eB);
-        }
+        if (t < y_hi) load_e(t, eA);
+        beat(I1(), t + 1, inB, eB);
+        if (t + 3 < y_hi) load_psi(t + 3, inB);
+        if (t + 1 < y_hi) load_e(t + 1, eB);
     }
 
     if (IMAG) {
@@ -455,6 +551,49 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
                 __hip_atomic_store(p.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// periodic halo exchange
+// ---------------------------------------------------------------------------
+// The single-process exchange of the reference (src/cpukernel.cpp:707-715,
+// then :738-746): first the x halo of rows [rx0, rx0 + H - 2*halo_y) (that row
+// count holds even on a closed y axis) is copied from the opposite inner edge,
+// then the full-width y halo rows.  Every destination is computed from its final
+// source directly (a y-halo cell whose source row had its x halo refreshed
+// takes the inner cell both copies lead to), so sources are never destinations
+// and one launch suffices.
+template <typename T>
+__global__ void __launch_bounds__(256) halo_exchange_kernel(T *re, T *im, int W, int H, int pitch, int px, int py,
+                                                            int hx, int hy) {
+    const int nx = W - 2 * hx, ny = H - 2 * hy;
+    const int rx0 = py ? hy : 0, rx1 = rx0 + max(0, H - 2 * hy);
+    // destinations: [0,hy) and [H-hy,H) full rows (py); x bands of other rows (px)
+    const int n_yband = py ? 2 * hy * W : 0;
+    const int n_xband = px ? 2 * hx * (H - (py ? 2 * hy : 0)) : 0;
+    const int n = n_yband + n_xband;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int x, y;
+        if (i < n_yband) {
+            const int r = i / W;
+            x = i - r * W;
+            y = r < hy ? r : H - 2 * hy + r;
+        } else {
+            const int j = i - n_yband;
+            const int r = j / (2 * hx);
+            const int c = j - r * 2 * hx;
+            y = (py ? hy : 0) + r;
+            x = c < hx ? c : W - 2 * hx + c;
+        }
+        int sy = y, sx = x;
+        if (py && (y < hy || y >= H - hy)) sy = y < hy ? y + ny : y - ny;
+        const bool xd = px && (x < hx || x >= W - hx) && sy >= rx0 && sy < rx1;
+        if (xd) sx = x < hx ? x + nx : x - nx;
+        if (sx == x && sy == y) continue;
+        const long d = (long)y * pitch + x, s = (long)sy * pitch + sx;
+        re[d] = re[s];
+        im[d] = im[s];
     }
 }
 

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -57,6 +58,7 @@ struct tsmi_ctx {
     bool div_pending = false, has_state = false, has_e = false, has_v = false;
     hipStream_t own = nullptr, stream = nullptr;
     double aH = 0, bH = 0, aV = 0, bV = 0, c_nl = 0, c_lhy = 0;
+    double corr_re = 1.0, corr_im = 0.0;   // FP32 real time: folded into E
     int ix0 = 0, ix1 = 0, iy0 = 0, iy1 = 0;
 };
 
@@ -114,7 +116,55 @@ int current_div(tsmi_ctx *c, double *out) {
     return TSMI_OK;
 }
 
-template <typename T, bool IMAG, int VEC>
+// W x H FP64 (row stride W) -> pitched plane of T, rounding once.
+template <typename T>
+__global__ void import_plane(const double *__restrict__ src, T *__restrict__ dst, int W, int H, int pitch) {
+    const long n = (long)W * H;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long y = i / W, x = i - y * W;
+        dst[y * pitch + x] = (T)src[i];
+    }
+}
+
+// pitched plane of T -> W x H FP64, divided by *div when given
+template <typename T>
+__global__ void export_plane(const T *__restrict__ src, double *__restrict__ dst, int W, int H, int pitch,
+                             const double *__restrict__ div) {
+    const long n = (long)W * H;
+    const double d = div ? *div : 1.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long y = i / W, x = i - y * W;
+        double v = (double)src[y * pitch + x];
+        if (div) v /= d;
+        dst[i] = v;
+    }
+}
+
+// E table for the FP32 real-time step: E*corr computed in FP64, each component
+// then rounded once to the nearest float.  corr undoes, for the whole step, the
+// FP32 quantisation of the kinetic pair coefficients (see tsmi_create).
+__global__ void import_exp_f32(const double *__restrict__ sre, const double *__restrict__ sim, double cr, double ci,
+                               float *__restrict__ dre, float *__restrict__ dim, int W, int H, int pitch) {
+    const long n = (long)W * H;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long y = i / W, x = i - y * W;
+        const double er = sre[i], ei = sim[i];
+        dre[y * pitch + x] = (float)(er * cr - ei * ci);
+        dim[y * pitch + x] = (float)(er * ci + ei * cr);
+    }
+}
+
+int import_device(tsmi_ctx *c, void *dst, const double *src) {
+    const int blocks = 2048;
+    if (c->esz == 8)
+        hipLaunchKernelGGL(import_plane<double>, dim3(blocks), dim3(256), 0, c->stream, src, (double *)dst, c->W, c->H, c->pitch);
+    else
+        hipLaunchKernelGGL(import_plane<float>, dim3(blocks), dim3(256), 0, c->stream, src, (float *)dst, c->W, c->H, c->pitch);
+    TSMI_HIP(hipGetLastError());
+    return TSMI_OK;
+}
+
+template <typename T, bool IMAG, int VEC, bool LHY>
 int launch_step(tsmi_ctx *c, int exchange) {
     StepParams<T> p;
     std::memset(&p, 0, sizeof(p));
@@ -126,12 +176,9 @@ int launch_step(tsmi_ctx *c, int exchange) {
     p.e_re = (const T *)c->e_re;
     p.e_im = (const T *)c->e_im;
     p.W = c->W; p.H = c->H; p.pitch = c->pitch;
-    p.px = c->d.periodic_x; p.py = c->d.periodic_y; p.hx = c->d.halo_x; p.hy = c->d.halo_y;
-    p.exchange = exchange;
     p.n_strips = c->n_strips; p.n_waves = c->n_waves; p.seg_rows = c->seg_rows;
     p.aH = (T)c->aH; p.bH = (T)c->bH; p.aV = (T)c->aV; p.bV = (T)c->bV;
     p.c_nl = (T)c->c_nl; p.c_lhy = (T)c->c_lhy;
-    p.use_lhy = c->c_lhy != 0.0;
     p.ix0 = c->ix0; p.ix1 = c->ix1; p.iy0 = c->iy0; p.iy1 = c->iy1;
     if (IMAG) {
         p.div_in = c->div_pending ? c->div + c->div_slot : nullptr;
@@ -143,8 +190,16 @@ int launch_step(tsmi_ctx *c, int exchange) {
     }
     // the grid covers every (strip, segment) pair exactly once
     const int blocks = (c->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
-    hipLaunchKernelGGL((step_kernel<T, IMAG, VEC>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
+    hipLaunchKernelGGL((step_kernel<T, IMAG, VEC, LHY>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
     TSMI_HIP(hipGetLastError());
+    if (exchange && (c->d.periodic_x || c->d.periodic_y)) {
+        const long cells = 2L * c->d.halo_y * c->W + 2L * c->d.halo_x * c->H;
+        const int eb = (int)std::min<long>(1024, (cells + 255) / 256);
+        hipLaunchKernelGGL((halo_exchange_kernel<T>), dim3(eb), dim3(256), 0, c->stream, (T *)c->psi_re[nb],
+                           (T *)c->psi_im[nb], c->W, c->H, c->pitch, c->d.periodic_x, c->d.periodic_y,
+                           c->d.halo_x, c->d.halo_y);
+        TSMI_HIP(hipGetLastError());
+    }
     c->cur = nb;
     if (IMAG) {
         c->div_slot = 1 - c->div_slot;
@@ -153,10 +208,17 @@ int launch_step(tsmi_ctx *c, int exchange) {
     return TSMI_OK;
 }
 
+template <typename T, int VEC>
+int step_typed(tsmi_ctx *c, int exchange) {
+    // the LHY term is compiled in only when g_LHY != 0 (it is exactly zero otherwise)
+    const bool lhy = c->c_lhy != 0.0;
+    if (c->d.imag_time)
+        return lhy ? launch_step<T, true, VEC, true>(c, exchange) : launch_step<T, true, VEC, false>(c, exchange);
+    return lhy ? launch_step<T, false, VEC, true>(c, exchange) : launch_step<T, false, VEC, false>(c, exchange);
+}
+
 int step_once(tsmi_ctx *c, int exchange) {
-    if (c->esz == 4)
-        return c->d.imag_time ? launch_step<float, true, 4>(c, exchange) : launch_step<float, false, 4>(c, exchange);
-    return c->d.imag_time ? launch_step<double, true, 2>(c, exchange) : launch_step<double, false, 2>(c, exchange);
+    return c->esz == 4 ? step_typed<float, 4>(c, exchange) : step_typed<double, 2>(c, exchange);
 }
 
 template <typename T>
@@ -260,6 +322,29 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     }
     c->c_nl = d.coupling * d.delta_t;
     c->c_lhy = d.coupling_lhy * d.delta_t;
+    if (c->esz == 4) {
+        // The FP32 step uses the nearest floats of the pair coefficients.  A pair
+        // factor [[a, ib], [ib, a]] = a + i b sigma_x is sqrt(a^2+b^2) times a
+        // unitary matrix whatever the mode, and multiplies smooth modes by a + ib.
+        // Rounded, |a+ib| misses 1 by ~1e-8 and arg(a+ib) misses beta by ~1e-9;
+        // a step repeats that 4 times per axis, so after 1000 steps the norm would
+        // drift by ~4e-4.  The factor corr = (e^{i beta} / (a_f + i b_f))^4 per
+        // axis restores the modulus exactly for every mode and the phase of smooth
+        // modes; it is folded into E (real time only: in imaginary time the
+        // renormalisation absorbs any uniform scale).
+        const float faH = (float)c->aH, fbH = (float)c->bH, faV = (float)c->aV, fbV = (float)c->bV;
+        std::complex<double> corr(1.0, 0.0);
+        if (!d.imag_time) {
+            const std::complex<double> zH(faH, fbH), zV(faV, fbV);
+            const std::complex<double> eH(std::cos(bx), std::sin(bx)), eV(std::cos(by), std::sin(by));
+            const std::complex<double> qH = eH / zH, qV = eV / zV;
+            corr = qH * qH * qH * qH;
+            if (c->H > 1) corr *= qV * qV * qV * qV;
+        }
+        c->corr_re = corr.real();
+        c->corr_im = corr.imag();
+        c->aH = faH; c->bH = fbH; c->aV = faV; c->bV = fbV;
+    }
 
     auto bail = [&](int code, const std::string &m) {
         free_all(c);
@@ -306,7 +391,7 @@ void tsmi_destroy(tsmi_ctx *c) {
 int tsmi_set_stream(tsmi_ctx *c, void *s) {
     if (!c) return fail(TSMI_E_INVALID, "tsmi_set_stream: null context");
     TSMI_HIP(hipStreamSynchronize(c->stream));
-    c->stream = s ? (hipStream_t)s : c->own;
+    c->stream = (hipStream_t)s;
     return TSMI_OK;
 }
 
@@ -320,9 +405,81 @@ int tsmi_set_state(tsmi_ctx *c, const double *re, const double *im) {
     return TSMI_OK;
 }
 
+int tsmi_set_state_device(tsmi_ctx *c, const double *re, const double *im) {
+    if (!c || !re || !im) return fail(TSMI_E_INVALID, "tsmi_set_state_device: null argument");
+    int st = import_device(c, c->psi_re[c->cur], re);
+    if (!st) st = import_device(c, c->psi_im[c->cur], im);
+    if (st) return st;
+    c->has_state = true;
+    c->div_pending = false;
+    return TSMI_OK;
+}
+
+int tsmi_set_exp_potential_device(tsmi_ctx *c, const double *e_re, const double *e_im) {
+    if (!c || !e_re) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential_device: null argument");
+    if (!c->d.imag_time && !e_im) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential_device: real time needs Im(E)");
+    if (c->esz == 4 && !c->d.imag_time) {
+        hipLaunchKernelGGL(import_exp_f32, dim3(2048), dim3(256), 0, c->stream, e_re, e_im, c->corr_re, c->corr_im,
+                           (float *)c->e_re, (float *)c->e_im, c->W, c->H, c->pitch);
+        TSMI_HIP(hipGetLastError());
+        c->has_e = true;
+        return TSMI_OK;
+    }
+    int st = import_device(c, c->e_re, e_re);
+    if (!st && e_im) st = import_device(c, c->e_im, e_im);
+    if (st) return st;
+    c->has_e = true;
+    return TSMI_OK;
+}
+
+int tsmi_set_potential_device(tsmi_ctx *c, const double *v) {
+    if (!c || !v) return fail(TSMI_E_INVALID, "tsmi_set_potential_device: null argument");
+    TSMI_HIP(hipMemcpy2DAsync(c->v, (size_t)c->pitch * 8, v, (size_t)c->W * 8, (size_t)c->W * 8, c->H,
+                              hipMemcpyDeviceToDevice, c->stream));
+    c->has_v = true;
+    return TSMI_OK;
+}
+
+int tsmi_get_state_device(tsmi_ctx *c, double *re, double *im) {
+    if (!c || !re || !im) return fail(TSMI_E_INVALID, "tsmi_get_state_device: null argument");
+    if (!c->has_state) return fail(TSMI_E_INVALID, "tsmi_get_state_device: no state");
+    const double *div = (c->d.imag_time && c->div_pending) ? c->div + c->div_slot : nullptr;
+    const int blocks = 2048;
+    for (int k = 0; k < 2; ++k) {
+        const void *src = k ? c->psi_im[c->cur] : c->psi_re[c->cur];
+        double *dst = k ? im : re;
+        if (c->esz == 8)
+            hipLaunchKernelGGL(export_plane<double>, dim3(blocks), dim3(256), 0, c->stream, (const double *)src, dst,
+                               c->W, c->H, c->pitch, div);
+        else
+            hipLaunchKernelGGL(export_plane<float>, dim3(blocks), dim3(256), 0, c->stream, (const float *)src, dst,
+                               c->W, c->H, c->pitch, div);
+        TSMI_HIP(hipGetLastError());
+    }
+    return TSMI_OK;
+}
+
 int tsmi_set_exp_potential(tsmi_ctx *c, const double *e_re, const double *e_im) {
     if (!c || !e_re) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential: null argument");
     if (!c->d.imag_time && !e_im) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential: real time needs Im(E)");
+    if (c->esz == 4 && !c->d.imag_time) {
+        // stage the FP64 table on the device and apply the correction there
+        const size_t bytes = (size_t)c->W * (size_t)c->H * sizeof(double);
+        double *tmp = nullptr;
+        TSMI_HIP(hipMalloc(&tmp, 2 * bytes));
+        hipError_t e = hipMemcpyAsync(tmp, e_re, bytes, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(tmp + bytes / 8, e_im, bytes, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(import_exp_f32, dim3(2048), dim3(256), 0, c->stream, tmp, tmp + bytes / 8,
+                               c->corr_re, c->corr_im, (float *)c->e_re, (float *)c->e_im, c->W, c->H, c->pitch);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(TSMI_E_HIP, std::string("exp potential upload: ") + hipGetErrorString(e));
+        c->has_e = true;
+        return TSMI_OK;
+    }
     int st = upload_plane(c, c->e_re, e_re);
     if (!st && e_im) st = upload_plane(c, c->e_im, e_im);
     if (st) return st;

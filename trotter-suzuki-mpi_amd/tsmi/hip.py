@@ -82,6 +82,22 @@ class HipKernel:
         v = self._plane(v)
         _lib.check(self._L.tsmi_set_potential(self._h, _lib.dptr(v)), "tsmi_set_potential")
 
+    # -- device-resident data (pointers to W*H FP64 arrays, e.g. torch tensors) ----
+    def set_state_device(self, re_ptr: int, im_ptr: int) -> None:
+        _lib.check(self._L.tsmi_set_state_device(self._h, C.c_void_p(re_ptr), C.c_void_p(im_ptr)),
+                   "tsmi_set_state_device")
+
+    def set_exp_potential_device(self, e_re_ptr: int, e_im_ptr: int | None) -> None:
+        _lib.check(self._L.tsmi_set_exp_potential_device(self._h, C.c_void_p(e_re_ptr), C.c_void_p(e_im_ptr or 0)),
+                   "tsmi_set_exp_potential_device")
+
+    def set_potential_device(self, v_ptr: int) -> None:
+        _lib.check(self._L.tsmi_set_potential_device(self._h, C.c_void_p(v_ptr)), "tsmi_set_potential_device")
+
+    def get_state_device(self, re_ptr: int, im_ptr: int) -> None:
+        _lib.check(self._L.tsmi_get_state_device(self._h, C.c_void_p(re_ptr), C.c_void_p(im_ptr)),
+                   "tsmi_get_state_device")
+
     def set_stream(self, stream_handle: int | None) -> None:
         _lib.check(self._L.tsmi_set_stream(self._h, C.c_void_p(stream_handle or 0)), "tsmi_set_stream")
 
