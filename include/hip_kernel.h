@@ -1,0 +1,61 @@
+/*
+ * hip_kernel.h -- HipKernel, the MI355X implementation of ITrotterKernel.
+ *
+ * HipKernel is a sibling of the reference's CPUBlock (src/kernel.h:66-151) with
+ * the same constructor arguments (src/kernel.h:68-70), so the reference's kernel
+ * factory (Solver::init_kernel, src/solver.cpp:118-148) can build it for a new
+ * kernel type.  It compiles against the reference's own trottersuzuki.h or
+ * against this project's include/trottersuzuki.h: it only uses the members both
+ * declare.  All device work goes through the C ABI of libtsmi_hip.so
+ * (include/tsmi_hip.h); failures throw std::runtime_error, as my_abort does in
+ * a build without MPI (src/common.cpp:104-115).
+ *
+ * Supported: one component, Cartesian coordinates, angular velocity 0, real or
+ * imaginary time, closed or periodic axes, FP64 or FP32 bulk.
+ */
+#ifndef TSMI_HIP_KERNEL_H
+#define TSMI_HIP_KERNEL_H
+
+#include <string>
+
+// angle brackets: the include path decides which trottersuzuki.h is used
+#include <trottersuzuki.h>
+#include <tsmi_hip.h>
+
+class HipKernel : public ITrotterKernel {
+public:
+    /** precision: 64 (FP64 throughout) or 32 (FP32 bulk, FP64 reductions). */
+    HipKernel(Lattice *grid, State *state, Hamiltonian *hamiltonian, double *external_pot_real,
+              double *external_pot_imag, double delta_t, double norm, bool imag_time, int precision = 64);
+    ~HipKernel();
+
+    void run_kernel();                 ///< one step; the halo exchange if start_halo_exchange() preceded it
+    void run_kernel_on_halo();         ///< no-op: the step kernel covers the whole tile
+    void wait_for_completion();        ///< no-op: the renormalisation is part of the step
+    void get_sample(size_t dest_stride, size_t x, size_t y, size_t width, size_t height,
+                    double *dest_real, double *dest_imag, double *dest_real2 = 0, double *dest_imag2 = 0) const;
+    void normalization();              ///< two components only: throws
+    void rabi_coupling(double var, double delta_t);   ///< two components only: throws
+    double calculate_squared_norm(bool global = true) const;
+    bool runs_in_place() const { return false; }
+    std::string get_name() const { return precision_ == 32 ? "HIP-FP32" : "HIP"; }
+    void update_potential(double *_external_pot_real, double *_external_pot_imag, int which);
+    void cpy_first_positive_to_first_negative() {}    ///< cylindrical only: nothing to do
+    void start_halo_exchange();        ///< marks the next run_kernel() as followed by the exchange
+    void finish_halo_exchange() {}     ///< the exchange runs on the device right after the step
+
+    // Beyond the interface: used by this project's Solver.
+    void evolve(int n_steps);          ///< n steps, the exchange after all but the last
+    void set_potential(const double *v);   ///< raw V on the tile, for energy()
+    void energy(double out[6]) const;  ///< total, kinetic, potential, intra, LHY, squared norm
+    void expected_values(double out[10]) const;
+    tsmi_ctx *context() const { return ctx_; }
+
+private:
+    tsmi_ctx *ctx_;
+    int precision_;
+    int width_, height_;
+    bool exchange_next_;
+};
+
+#endif

@@ -100,6 +100,12 @@ int tsmi_set_potential_device(tsmi_ctx *ctx, const double *v);
  * wait_for_completion (:551-570).  Asynchronous on the context stream. */
 int tsmi_evolve(tsmi_ctx *ctx, int32_t n_steps);
 
+/* One step, followed by the periodic halo exchange when exchange != 0:
+ * run_kernel_on_halo + run_kernel with start/finish_halo_exchange
+ * (src/cpukernel.cpp:413-524, :691-747) and, in imaginary time,
+ * wait_for_completion's renormalisation (:551-570).  Asynchronous. */
+int tsmi_step(tsmi_ctx *ctx, int32_t exchange);
+
 /* Squared norm of the inner region times dx*dy:
  * ITrotterKernel::calculate_squared_norm (src/cpukernel.cpp:526-549).  Syncs. */
 int tsmi_squared_norm(tsmi_ctx *ctx, double *out);

@@ -1,0 +1,87 @@
+// dropin_check.cpp -- HipKernel inside the reference's own object model.
+//
+// TEST INFRASTRUCTURE ONLY (built into oracle/_ref/ from the reference sources,
+// run by tests/test_cpp_api.py on the GPU host).  Compiled against the
+// reference's trottersuzuki.h, it builds the reference's Lattice2D, State,
+// Potential and Hamiltonian from a case file (the format of ref_driver.cpp),
+// constructs HipKernel with the arguments the reference's kernel factory passes
+// to CPUBlock (src/solver.cpp:124), drives it with the reference's per-step
+// protocol (src/solver.cpp:186-239) and compares the result with the reference
+// Solver's own "cpu" run of the same case.
+//
+//   dropin_check case.bin [precision]   ->  one JSON line
+#include <trottersuzuki.h>
+#include <hip_kernel.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+static void die(const char *m) { std::fprintf(stderr, "dropin_check: %s\n", m); std::exit(2); }
+
+int main(int argc, char **argv) {
+    if (argc < 2) die("usage: dropin_check case.bin [precision]");
+    const int precision = argc > 2 ? std::atoi(argv[2]) : 64;
+    FILE *f = std::fopen(argv[1], "rb");
+    if (!f) die("cannot open case");
+    int32_t h[10];
+    double par[6];
+    if (std::fread(h, 4, 10, f) != 10 || std::fread(par, 8, 6, f) != 6) die("short header");
+    Lattice2D grid(h[2], par[0], h[3], par[1], h[4] != 0, h[5] != 0);
+    const size_t n = (size_t)grid.dim_x * grid.dim_y;
+    std::vector<double> re(n), im(n), v(n);
+    if (std::fread(re.data(), 8, n, f) != n || std::fread(im.data(), 8, n, f) != n || std::fread(v.data(), 8, n, f) != n)
+        die("short data");
+    std::fclose(f);
+    const int steps = h[6];
+    const bool imag = h[7] != 0;
+    const double dt = par[2];
+
+    // reference Solver on a copy
+    std::vector<double> cre(re), cim(im);
+    State cstate(&grid, 0, cre.data(), cim.data());
+    Potential pot(&grid, v.data());
+    Hamiltonian ham(&grid, &pot, par[3], par[4], par[5]);
+    Solver solver(&grid, &cstate, &ham, dt, "cpu");
+    solver.evolve(steps, imag);
+
+    // the same evolution through HipKernel, as Solver::evolve drives a kernel
+    State hstate(&grid, 0, re.data(), im.data());
+    std::vector<double> ere(n), eim(n);
+    for (int y = 0; y < grid.dim_y; ++y)
+        for (int x = 0; x < grid.dim_x; ++x) {
+            const double p = pot.get_value(x, y);
+            const std::complex<double> e = imag ? std::exp(std::complex<double>(-dt * p, 0.))
+                                                : std::exp(std::complex<double>(0., -dt * p));
+            ere[(size_t)y * grid.dim_x + x] = e.real();
+            eim[(size_t)y * grid.dim_x + x] = e.imag();
+        }
+    const double norm = imag ? hstate.get_squared_norm() : 0.;
+    ITrotterKernel *k = new HipKernel(&grid, &hstate, &ham, ere.data(), eim.data(), dt, norm, imag, precision);
+    for (int i = 0; i < steps; ++i) {
+        k->run_kernel_on_halo();
+        if (i != steps - 1) k->start_halo_exchange();
+        k->run_kernel();
+        if (i != steps - 1) k->finish_halo_exchange();
+        k->wait_for_completion();
+        k->cpy_first_positive_to_first_negative();
+    }
+    std::vector<double> ore(n), oim(n);
+    k->get_sample(grid.dim_x, 0, 0, grid.dim_x, grid.dim_y, ore.data(), oim.data());
+    const double kn = k->calculate_squared_norm();
+    const std::string name = k->get_name();
+    delete k;
+
+    double num = 0, den = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const double dr = ore[i] - cre[i], di = oim[i] - cim[i];
+        num += dr * dr + di * di;
+        den += cre[i] * cre[i] + cim[i] * cim[i];
+    }
+    std::printf("{\"kernel\": \"%s\", \"rel_l2\": %.6e, \"kernel_norm\": %.17g, \"solver_norm\": %.17g}\n",
+                name.c_str(), std::sqrt(num / den), kn, solver.get_squared_norm());
+    return 0;
+}

@@ -107,7 +107,9 @@ def test_fp64_matches_reference_golden(name, oracle):
     psi, e, x, _ = run_hip(wl, oracle, steps=int(z["steps"]))
     ref = z["psi_re"] + 1j * z["psi_im"]
     assert rel_l2(psi, ref) < 1e-12
-    np.testing.assert_allclose(e, z["energies"], rtol=1e-12, atol=1e-14)
+    # the reference sums its cells in plain sequence (relative error up to
+    # ~N eps = 1.5e-11 for 65536 cells); this library sums with compensation
+    np.testing.assert_allclose(e, z["energies"], rtol=1e-11, atol=1e-14)
     np.testing.assert_allclose(x, z["expect"], rtol=1e-11, atol=1e-13)
 
 

@@ -48,7 +48,12 @@ struct StepParams {
     const T *e_re, *e_im;        // E = exp(-i dt V) planes (e_im unused in imaginary time)
     int W, H, pitch;             // tile and row pitch (elements)
     int n_strips, n_waves, seg_rows;
-    T aH, bH, aV, bV, c_nl, c_lhy;
+    // Pair factors in the scaled frame (see the note at fpair): a pair becomes
+    // p' = m p + i t q (real time) or m p + t q (imaginary time), with (m, t) =
+    // (1, b/a) in mode A and (a/b, 1) in mode B; a cell without a partner in a
+    // pass is multiplied by un (= 1/g of that axis).
+    T mH, tH, mV, tV, unH, unV;
+    T c_nl, c_lhy;               // nonlinear couplings, already rescaled to the frame
     // imaginary time only
     const double *div_in;        // divisor of the previous renormalisation (nullptr: 1)
     double *div_out;             // divisor computed from this step's output
@@ -92,6 +97,48 @@ __device__ __forceinline__ void rot_q(T a, T b, T pr, T pi, T &qr, T &qi) {
     else       { qr = a * qr + b * pr; qi = a * qi + b * pi; }
 }
 
+// Scaled frame.  The exact pair factor a I + i b sigma_x equals s (m I + i t sigma_x)
+// with s = a (mode A, |a| >= |b|) or s = b (mode B); the kernel applies only the
+// bracket, whose one coefficient is 1, so each output is a single fused
+// multiply-add.  The dropped per-pass scales are uniform, so the host folds
+// their product for a whole step into E and rescales the nonlinear coupling to
+// the density seen mid-step; with the coefficients rounded to the kernel's
+// precision it uses g = (exact eigenvalue)/(computed eigenvalue) of the smooth
+// modes per pass, which keeps the modulus exact for every mode.
+template <bool IMAG, int MODE, typename T>
+__device__ __forceinline__ void fpair(T k, T &pr, T &pi, T &qr, T &qi) {
+    const T r0 = pr, i0 = pi;
+    if (!IMAG) {
+        if (MODE == 0) {          // p + i k q
+            pr = fma(-k, qi, r0); pi = fma(k, qr, i0);
+            qr = fma(-k, i0, qr); qi = fma(k, r0, qi);
+        } else {                  // k p + i q
+            pr = fma(k, r0, -qi); pi = fma(k, i0, qr);
+            qr = fma(k, qr, -i0); qi = fma(k, qi, r0);
+        }
+    } else {                      // p + k q (mode A only)
+        pr = fma(k, qr, r0); pi = fma(k, qi, i0);
+        qr = fma(k, r0, qr); qi = fma(k, i0, qi);
+    }
+}
+
+template <bool IMAG, int MODE, typename T>
+__device__ __forceinline__ void fpair_p(T k, T &pr, T &pi, T qr, T qi) {
+    const T r0 = pr, i0 = pi;
+    if (!IMAG) {
+        if (MODE == 0) { pr = fma(-k, qi, r0); pi = fma(k, qr, i0); }
+        else           { pr = fma(k, r0, -qi); pi = fma(k, i0, qr); }
+    } else { pr = fma(k, qr, r0); pi = fma(k, qi, i0); }
+}
+
+template <bool IMAG, int MODE, typename T>
+__device__ __forceinline__ void fpair_q(T k, T pr, T pi, T &qr, T &qi) {
+    if (!IMAG) {
+        if (MODE == 0) { qr = fma(-k, pi, qr); qi = fma(k, pr, qi); }
+        else           { qr = fma(k, qr, -pi); qi = fma(k, qi, pr); }
+    } else { qr = fma(k, pr, qr); qi = fma(k, pi, qi); }
+}
+
 template <typename T>
 __device__ __forceinline__ T shfl_from_next(T v) { return __shfl_down(v, 1, kWave); }
 template <typename T>
@@ -107,25 +154,52 @@ struct HCoef {
 };
 
 template <typename T, int VEC>
-__device__ __forceinline__ HCoef<T, VEC> make_hcoef(int xl, int lane, int W, T a, T b) {
+__device__ __forceinline__ HCoef<T, VEC> make_hcoef(int xl, int lane, int W, T a, T b, T un) {
     HCoef<T, VEC> c;
     auto ok = [&](int x) { return x >= 0 && x + 1 < W; };
 #pragma unroll
     for (int k = 0; k < VEC; k += 2) {
         const bool v = ok(xl + k);
-        c.ae[k / 2] = v ? a : T(1);
+        c.ae[k / 2] = v ? a : un;
         c.be[k / 2] = v ? b : T(0);
     }
 #pragma unroll
     for (int k = 1; k < VEC; k += 2) {
         const bool v = ok(xl + k) && (k + 1 < VEC || lane < kWave - 1);
-        c.ao[k / 2] = v ? a : T(1);
+        c.ao[k / 2] = v ? a : un;
         c.bo[k / 2] = v ? b : T(0);
     }
     const bool vl = ok(xl - 1) && lane > 0;
-    c.al = vl ? a : T(1);
+    c.al = vl ? a : un;
     c.bl = vl ? b : T(0);
     return c;
+}
+
+// H(s) in the fast form: every pair of the wave is inside the tile.
+template <int PAR, bool IMAG, int MODE, typename T, int VEC>
+__device__ __forceinline__ void hpass_fast(Row<T, VEC> &r, T k) {
+    if constexpr (PAR == 0) {
+#pragma unroll
+        for (int j = 0; j < VEC; j += 2) fpair<IMAG, MODE>(k, r.re[j], r.im[j], r.re[j + 1], r.im[j + 1]);
+    } else {
+        const T nr = shfl_from_next(r.re[0]), ni = shfl_from_next(r.im[0]);
+        const T pr = shfl_from_prev(r.re[VEC - 1]), pi = shfl_from_prev(r.im[VEC - 1]);
+#pragma unroll
+        for (int j = 1; j + 1 < VEC; j += 2) fpair<IMAG, MODE>(k, r.re[j], r.im[j], r.re[j + 1], r.im[j + 1]);
+        fpair_p<IMAG, MODE>(k, r.re[VEC - 1], r.im[VEC - 1], nr, ni);
+        fpair_q<IMAG, MODE>(k, pr, pi, r.re[0], r.im[0]);
+    }
+}
+
+template <int PAR, bool IMAG, int MODE, typename T, int VEC>
+__device__ __forceinline__ void vpair_fast(Row<T, VEC> &u, Row<T, VEC> &l, T k) {
+    if constexpr (PAR == 0) {
+#pragma unroll
+        for (int j = 0; j < VEC; j += 2) fpair<IMAG, MODE>(k, u.re[j], u.im[j], l.re[j], l.im[j]);
+    } else {
+#pragma unroll
+        for (int j = 1; j < VEC; j += 2) fpair<IMAG, MODE>(k, u.re[j], u.im[j], l.re[j], l.im[j]);
+    }
 }
 
 // H(s) on row y: pairs (x, x+1) with x+y+s even; x has the parity of k, so the
@@ -369,7 +443,7 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
 // ---------------------------------------------------------------------------
 // the kernel
 // ---------------------------------------------------------------------------
-template <typename T, bool IMAG, int VEC, bool LHY>
+template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
 __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform by construction; readfirstlane lets the compiler keep every
@@ -391,8 +465,12 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
     // lanes that store: not in the apron, inside the tile
     const bool own = lane >= kApron / VEC && lane < kWave - kApron / VEC;
 
-    const HCoef<T, VEC> hc = make_hcoef<T, VEC>(xl, lane, p.W, p.aH, p.bH);
-    const T aV = p.aV, bV = p.bV;
+    const HCoef<T, VEC> hc = make_hcoef<T, VEC>(xl, lane, p.W, p.mH, p.tH, p.unH);
+    const T mV = p.mV, tV = p.tV, unV = p.unV;
+    const T kH = MODE == 0 ? p.tH : p.mH, kV = MODE == 0 ? p.tV : p.mV;
+    // every horizontal pair of the wave lies inside the tile (MODE < 0: the two
+    // axes use different scaled forms, so only the general form runs)
+    const bool interior = MODE >= 0 && X0 >= 0 && X0 + kWave * VEC <= p.W;
 
     double divd = 1.0;
     if (IMAG && p.div_in) divd = *p.div_in;
@@ -475,10 +553,10 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
     auto beat = [&](auto tp, int t, Row<T, VEC> &in, const Row<T, VEC> &e) {
         constexpr int TP = decltype(tp)::value;
         const bool v0 = valid(t), v1 = valid(t - 1), v2 = valid(t - 2), v3 = valid(t - 3), v4 = valid(t - 4);
-        const T a1 = (v0 && v1) ? aV : T(1), b1 = (v0 && v1) ? bV : T(0);
-        const T a2 = (v1 && v2) ? aV : T(1), b2 = (v1 && v2) ? bV : T(0);
-        const T a3 = (v2 && v3) ? aV : T(1), b3 = (v2 && v3) ? bV : T(0);
-        const T a4 = (v3 && v4) ? aV : T(1), b4 = (v3 && v4) ? bV : T(0);
+        const T a1 = (v0 && v1) ? mV : unV, b1 = (v0 && v1) ? tV : T(0);
+        const T a2 = (v1 && v2) ? mV : unV, b2 = (v1 && v2) ? tV : T(0);
+        const T a3 = (v2 && v3) ? mV : unV, b3 = (v2 && v3) ? tV : T(0);
+        const T a4 = (v3 && v4) ? mV : unV, b4 = (v3 && v4) ? tV : T(0);
         // stage 1: V0, then H0 on row t-1
         vpair<(TP + 1) & 1, IMAG>(pend1, in, a1, b1);
         Row<T, VEC> o1 = pend1;
@@ -504,6 +582,31 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
         if (v4 && y >= y_out_lo && y < y_out_hi) emit(y, o4);
     };
 
+    // The same beat in the fast form: interior strip, all five rows valid.
+    auto beat_fast = [&](auto tp, int t, Row<T, VEC> &in, const Row<T, VEC> &e) {
+        constexpr int TP = decltype(tp)::value;
+        constexpr int M = MODE < 0 ? 0 : MODE;
+        vpair_fast<(TP + 1) & 1, IMAG, M>(pend1, in, kV);
+        Row<T, VEC> o1 = pend1;
+        pend1 = in;
+        hpass_fast<(TP + 1) & 1, IMAG, M>(o1, kH);
+        vpair_fast<(TP + 1) & 1, IMAG, M>(pend2, o1, kV);
+        Row<T, VEC> o2 = pend2;
+        pend2 = o1;
+        hpass_fast<(TP + 1) & 1, IMAG, M>(o2, kH);
+        ppass<IMAG, LHY>(o2, e, p.c_nl, p.c_lhy);
+        hpass_fast<(TP + 1) & 1, IMAG, M>(o2, kH);
+        vpair_fast<TP & 1, IMAG, M>(pend3, o2, kV);
+        Row<T, VEC> o3 = pend3;
+        pend3 = o2;
+        hpass_fast<(TP + 1) & 1, IMAG, M>(o3, kH);
+        vpair_fast<TP & 1, IMAG, M>(pend4, o3, kV);
+        const Row<T, VEC> o4 = pend4;
+        pend4 = o3;
+        const int y = t - 4;
+        if (y >= y_out_lo && y < y_out_hi) emit(y, o4);
+    };
+
     // Software pipeline: rows t+1, t+2 and E rows t-1, t are in flight while
     // row t runs.  A row that is not loaded (past the segment) keeps the finite
     // values of an older row; every pair touching it has coefficients (1, 0).
@@ -514,12 +617,21 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
     if (t + 1 < y_hi) load_psi(t + 1, inB);
     const int t_end = y_hi + 4;
     for (; t < t_end; t += 2) {
-        beat(I0(), t, inA, eA);
-        if (t + 2 < y_hi) load_psi(t + 2, inA);
-        if (t < y_hi) load_e(t, eA);
-        beat(I1(), t + 1, inB, eB);
-        if (t + 3 < y_hi) load_psi(t + 3, inB);
-        if (t + 1 < y_hi) load_e(t + 1, eB);
+        if (interior && t >= y_lo + 4 && t + 1 < y_hi) {
+            beat_fast(I0(), t, inA, eA);
+            if (t + 2 < y_hi) load_psi(t + 2, inA);
+            load_e(t, eA);
+            beat_fast(I1(), t + 1, inB, eB);
+            if (t + 3 < y_hi) load_psi(t + 3, inB);
+            load_e(t + 1, eB);
+        } else {
+            beat(I0(), t, inA, eA);
+            if (t + 2 < y_hi) load_psi(t + 2, inA);
+            if (t < y_hi) load_e(t, eA);
+            beat(I1(), t + 1, inB, eB);
+            if (t + 3 < y_hi) load_psi(t + 3, inB);
+            if (t + 1 < y_hi) load_e(t + 1, eB);
+        }
     }
 
     if (IMAG) {

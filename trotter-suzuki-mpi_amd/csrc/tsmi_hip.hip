@@ -57,8 +57,12 @@ struct tsmi_ctx {
     int cur = 0, div_slot = 0;
     bool div_pending = false, has_state = false, has_e = false, has_v = false;
     hipStream_t own = nullptr, stream = nullptr;
-    double aH = 0, bH = 0, aV = 0, bV = 0, c_nl = 0, c_lhy = 0;
-    double corr_re = 1.0, corr_im = 0.0;   // FP32 real time: folded into E
+    double aH = 0, bH = 0, aV = 0, bV = 0, c_nl = 0, c_lhy = 0;   // FP64, as the reference
+    // scaled-frame factors actually used by the kernel (values of the kernel precision)
+    double mH = 1, tH = 0, mV = 1, tV = 0, unH = 1, unV = 1;
+    double corr_re = 1.0, corr_im = 0.0;   // folded into E
+    double c_nl_s = 0, c_lhy_s = 0;        // couplings rescaled to the mid-step frame
+    int mode = 0;                          // 0 / 1: both axes in form A / B; -1: mixed
     int ix0 = 0, ix1 = 0, iy0 = 0, iy1 = 0;
 };
 
@@ -140,18 +144,30 @@ __global__ void export_plane(const T *__restrict__ src, double *__restrict__ dst
     }
 }
 
-// E table for the FP32 real-time step: E*corr computed in FP64, each component
-// then rounded once to the nearest float.  corr undoes, for the whole step, the
-// FP32 quantisation of the kinetic pair coefficients (see tsmi_create).
-__global__ void import_exp_f32(const double *__restrict__ sre, const double *__restrict__ sim, double cr, double ci,
-                               float *__restrict__ dre, float *__restrict__ dim, int W, int H, int pitch) {
+// E table of the step: E*corr computed in FP64, each component then rounded
+// once to the kernel precision.  corr is the step's product of the scaled-frame
+// factors (see tsmi_create).  sim may be null (imaginary time: E is real).
+template <typename T>
+__global__ void import_exp(const double *__restrict__ sre, const double *__restrict__ sim, double cr, double ci,
+                           T *__restrict__ dre, T *__restrict__ dim, int W, int H, int pitch) {
     const long n = (long)W * H;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const long y = i / W, x = i - y * W;
-        const double er = sre[i], ei = sim[i];
-        dre[y * pitch + x] = (float)(er * cr - ei * ci);
-        dim[y * pitch + x] = (float)(er * ci + ei * cr);
+        const double er = sre[i], ei = sim ? sim[i] : 0.0;
+        dre[y * pitch + x] = (T)(er * cr - ei * ci);
+        dim[y * pitch + x] = (T)(er * ci + ei * cr);
     }
+}
+
+int launch_import_exp(tsmi_ctx *c, const double *re, const double *im) {
+    if (c->esz == 4)
+        hipLaunchKernelGGL(import_exp<float>, dim3(2048), dim3(256), 0, c->stream, re, im, c->corr_re, c->corr_im,
+                           (float *)c->e_re, (float *)c->e_im, c->W, c->H, c->pitch);
+    else
+        hipLaunchKernelGGL(import_exp<double>, dim3(2048), dim3(256), 0, c->stream, re, im, c->corr_re, c->corr_im,
+                           (double *)c->e_re, (double *)c->e_im, c->W, c->H, c->pitch);
+    TSMI_HIP(hipGetLastError());
+    return TSMI_OK;
 }
 
 int import_device(tsmi_ctx *c, void *dst, const double *src) {
@@ -164,7 +180,7 @@ int import_device(tsmi_ctx *c, void *dst, const double *src) {
     return TSMI_OK;
 }
 
-template <typename T, bool IMAG, int VEC, bool LHY>
+template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
 int launch_step(tsmi_ctx *c, int exchange) {
     StepParams<T> p;
     std::memset(&p, 0, sizeof(p));
@@ -177,8 +193,9 @@ int launch_step(tsmi_ctx *c, int exchange) {
     p.e_im = (const T *)c->e_im;
     p.W = c->W; p.H = c->H; p.pitch = c->pitch;
     p.n_strips = c->n_strips; p.n_waves = c->n_waves; p.seg_rows = c->seg_rows;
-    p.aH = (T)c->aH; p.bH = (T)c->bH; p.aV = (T)c->aV; p.bV = (T)c->bV;
-    p.c_nl = (T)c->c_nl; p.c_lhy = (T)c->c_lhy;
+    p.mH = (T)c->mH; p.tH = (T)c->tH; p.mV = (T)c->mV; p.tV = (T)c->tV;
+    p.unH = (T)c->unH; p.unV = (T)c->unV;
+    p.c_nl = (T)c->c_nl_s; p.c_lhy = (T)c->c_lhy_s;
     p.ix0 = c->ix0; p.ix1 = c->ix1; p.iy0 = c->iy0; p.iy1 = c->iy1;
     if (IMAG) {
         p.div_in = c->div_pending ? c->div + c->div_slot : nullptr;
@@ -190,7 +207,7 @@ int launch_step(tsmi_ctx *c, int exchange) {
     }
     // the grid covers every (strip, segment) pair exactly once
     const int blocks = (c->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
-    hipLaunchKernelGGL((step_kernel<T, IMAG, VEC, LHY>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
+    hipLaunchKernelGGL((step_kernel<T, IMAG, VEC, LHY, MODE>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
     TSMI_HIP(hipGetLastError());
     if (exchange && (c->d.periodic_x || c->d.periodic_y)) {
         const long cells = 2L * c->d.halo_y * c->W + 2L * c->d.halo_x * c->H;
@@ -208,13 +225,38 @@ int launch_step(tsmi_ctx *c, int exchange) {
     return TSMI_OK;
 }
 
+template <typename T, bool IMAG, int VEC, bool LHY>
+hipError_t occupancy_of(int *blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, step_kernel<T, IMAG, VEC, LHY, 0>, kBlock, 0);
+}
+
+hipError_t occupancy_blocks(int precision, bool imag, bool lhy, int *blocks) {
+    if (precision == 32) {
+        if (imag) return lhy ? occupancy_of<float, true, 4, true>(blocks) : occupancy_of<float, true, 4, false>(blocks);
+        return lhy ? occupancy_of<float, false, 4, true>(blocks) : occupancy_of<float, false, 4, false>(blocks);
+    }
+    if (imag) return lhy ? occupancy_of<double, true, 2, true>(blocks) : occupancy_of<double, true, 2, false>(blocks);
+    return lhy ? occupancy_of<double, false, 2, true>(blocks) : occupancy_of<double, false, 2, false>(blocks);
+}
+
+template <typename T, bool IMAG, int VEC, bool LHY>
+int step_mode(tsmi_ctx *c, int exchange) {
+    if constexpr (IMAG) {
+        return launch_step<T, IMAG, VEC, LHY, 0>(c, exchange);   // form A on both axes
+    } else {
+        if (c->mode == 0) return launch_step<T, IMAG, VEC, LHY, 0>(c, exchange);
+        if (c->mode == 1) return launch_step<T, IMAG, VEC, LHY, 1>(c, exchange);
+        return launch_step<T, IMAG, VEC, LHY, -1>(c, exchange);
+    }
+}
+
 template <typename T, int VEC>
 int step_typed(tsmi_ctx *c, int exchange) {
     // the LHY term is compiled in only when g_LHY != 0 (it is exactly zero otherwise)
     const bool lhy = c->c_lhy != 0.0;
     if (c->d.imag_time)
-        return lhy ? launch_step<T, true, VEC, true>(c, exchange) : launch_step<T, true, VEC, false>(c, exchange);
-    return lhy ? launch_step<T, false, VEC, true>(c, exchange) : launch_step<T, false, VEC, false>(c, exchange);
+        return lhy ? step_mode<T, true, VEC, true>(c, exchange) : step_mode<T, true, VEC, false>(c, exchange);
+    return lhy ? step_mode<T, false, VEC, true>(c, exchange) : step_mode<T, false, VEC, false>(c, exchange);
 }
 
 int step_once(tsmi_ctx *c, int exchange) {
@@ -299,9 +341,19 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     c->n_strips = (c->W + sw - 1) / sw;
     int seg = d.tile_rows;
     if (seg <= 0) {
-        const long long target = 4096;   // wavefronts per launch: several per SIMD
-        seg = (int)std::max<long long>(16, (H * c->n_strips + target - 1) / target);
-        seg = (seg + 7) / 8 * 8;
+        // Size the launch to one round of resident wavefronts: as many row
+        // segments per strip as the device holds at this kernel's occupancy, so
+        // no partial second round idles most of the GPU.
+        seg = std::max<int>(16, (int)((H + 1) / 2));
+        int dev = 0, cus = 0, blocks_per_cu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            occupancy_blocks(d.precision, d.imag_time != 0, d.coupling_lhy != 0.0, &blocks_per_cu) == hipSuccess &&
+            cus > 0 && blocks_per_cu > 0) {
+            const long resident = (long)cus * blocks_per_cu * (kBlock / kWave);
+            const long segs = std::max<long>(1, resident / c->n_strips);
+            seg = (int)std::max<long>(16, (H + segs - 1) / segs);
+        }
     }
     seg = (seg + 1) / 2 * 2;   // segments start on even rows (the kernel's pair patterns assume it)
     c->seg_rows = std::min<int>(seg, c->H);
@@ -322,29 +374,40 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     }
     c->c_nl = d.coupling * d.delta_t;
     c->c_lhy = d.coupling_lhy * d.delta_t;
-    if (c->esz == 4) {
-        // The FP32 step uses the nearest floats of the pair coefficients.  A pair
-        // factor [[a, ib], [ib, a]] = a + i b sigma_x is sqrt(a^2+b^2) times a
-        // unitary matrix whatever the mode, and multiplies smooth modes by a + ib.
-        // Rounded, |a+ib| misses 1 by ~1e-8 and arg(a+ib) misses beta by ~1e-9;
-        // a step repeats that 4 times per axis, so after 1000 steps the norm would
-        // drift by ~4e-4.  The factor corr = (e^{i beta} / (a_f + i b_f))^4 per
-        // axis restores the modulus exactly for every mode and the phase of smooth
-        // modes; it is folded into E (real time only: in imaginary time the
-        // renormalisation absorbs any uniform scale).
-        const float faH = (float)c->aH, fbH = (float)c->bH, faV = (float)c->aV, fbV = (float)c->bV;
-        std::complex<double> corr(1.0, 0.0);
-        if (!d.imag_time) {
-            const std::complex<double> zH(faH, fbH), zV(faV, fbV);
-            const std::complex<double> eH(std::cos(bx), std::sin(bx)), eV(std::cos(by), std::sin(by));
-            const std::complex<double> qH = eH / zH, qV = eV / zV;
-            corr = qH * qH * qH * qH;
-            if (c->H > 1) corr *= qV * qV * qV * qV;
-        }
-        c->corr_re = corr.real();
-        c->corr_im = corr.imag();
-        c->aH = faH; c->bH = fbH; c->aV = faV; c->bV = fbV;
-    }
+    // Scaled-frame factors per axis (the kernel's fpair): form A (m, t) = (1, b/a)
+    // when |a| >= |b| or in imaginary time, form B (a/b, 1) otherwise, both
+    // rounded to the kernel precision.  g = lambda / lambda~ compares the exact
+    // smooth-mode eigenvalue of a pass (a + ib, or a + b in imaginary time) with
+    // the computed one (m + it, or m + t); its modulus is exact for every mode.
+    // A step runs 4 passes per axis, 2 of them before the potential step, so
+    // E takes corr = gH^4 gV^4 and the couplings see |psi|^2 / |gH gV|^4.
+    struct Axis { double m, t, un; std::complex<double> g; int form; };
+    auto axis = [&](double a, double b) {
+        Axis x;
+        x.form = (d.imag_time || std::fabs(a) >= std::fabs(b)) ? 0 : 1;
+        double m = x.form == 0 ? 1.0 : a / b, t = x.form == 0 ? b / a : 1.0;
+        if (c->esz == 4) { m = (float)m; t = (float)t; }
+        x.m = m; x.t = t;
+        const std::complex<double> lam = d.imag_time ? std::complex<double>(a + b, 0.0) : std::complex<double>(a, b);
+        const std::complex<double> lt = d.imag_time ? std::complex<double>(m + t, 0.0) : std::complex<double>(m, t);
+        x.g = lam / lt;
+        double un = 1.0 / x.g.real();
+        if (c->esz == 4) un = (float)un;
+        x.un = un;
+        return x;
+    };
+    const Axis ah = axis(c->aH, c->bH);
+    const Axis av = c->H > 1 ? axis(c->aV, c->bV) : Axis{1.0, 0.0, 1.0, std::complex<double>(1.0, 0.0), ah.form};
+    c->mH = ah.m; c->tH = ah.t; c->unH = ah.un;
+    c->mV = av.m; c->tV = av.t; c->unV = av.un;
+    c->mode = ah.form == av.form ? ah.form : -1;
+    std::complex<double> corr = ah.g * ah.g * ah.g * ah.g;
+    if (c->H > 1) corr *= av.g * av.g * av.g * av.g;
+    c->corr_re = corr.real();
+    c->corr_im = d.imag_time ? 0.0 : corr.imag();
+    const double sn = std::pow(std::abs(ah.g) * std::abs(av.g), 4);
+    c->c_nl_s = c->c_nl * sn;
+    c->c_lhy_s = c->c_lhy * std::pow(sn, 1.5);
 
     auto bail = [&](int code, const std::string &m) {
         free_all(c);
@@ -418,15 +481,7 @@ int tsmi_set_state_device(tsmi_ctx *c, const double *re, const double *im) {
 int tsmi_set_exp_potential_device(tsmi_ctx *c, const double *e_re, const double *e_im) {
     if (!c || !e_re) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential_device: null argument");
     if (!c->d.imag_time && !e_im) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential_device: real time needs Im(E)");
-    if (c->esz == 4 && !c->d.imag_time) {
-        hipLaunchKernelGGL(import_exp_f32, dim3(2048), dim3(256), 0, c->stream, e_re, e_im, c->corr_re, c->corr_im,
-                           (float *)c->e_re, (float *)c->e_im, c->W, c->H, c->pitch);
-        TSMI_HIP(hipGetLastError());
-        c->has_e = true;
-        return TSMI_OK;
-    }
-    int st = import_device(c, c->e_re, e_re);
-    if (!st && e_im) st = import_device(c, c->e_im, e_im);
+    const int st = launch_import_exp(c, e_re, c->d.imag_time ? nullptr : e_im);
     if (st) return st;
     c->has_e = true;
     return TSMI_OK;
@@ -462,26 +517,18 @@ int tsmi_get_state_device(tsmi_ctx *c, double *re, double *im) {
 int tsmi_set_exp_potential(tsmi_ctx *c, const double *e_re, const double *e_im) {
     if (!c || !e_re) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential: null argument");
     if (!c->d.imag_time && !e_im) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential: real time needs Im(E)");
-    if (c->esz == 4 && !c->d.imag_time) {
-        // stage the FP64 table on the device and apply the correction there
-        const size_t bytes = (size_t)c->W * (size_t)c->H * sizeof(double);
-        double *tmp = nullptr;
-        TSMI_HIP(hipMalloc(&tmp, 2 * bytes));
-        hipError_t e = hipMemcpyAsync(tmp, e_re, bytes, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(tmp + bytes / 8, e_im, bytes, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(import_exp_f32, dim3(2048), dim3(256), 0, c->stream, tmp, tmp + bytes / 8,
-                               c->corr_re, c->corr_im, (float *)c->e_re, (float *)c->e_im, c->W, c->H, c->pitch);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        (void)hipFree(tmp);
-        if (e != hipSuccess) return fail(TSMI_E_HIP, std::string("exp potential upload: ") + hipGetErrorString(e));
-        c->has_e = true;
-        return TSMI_OK;
-    }
-    int st = upload_plane(c, c->e_re, e_re);
-    if (!st && e_im) st = upload_plane(c, c->e_im, e_im);
+    // stage the FP64 table on the device; the import kernel applies corr and rounds
+    const size_t n = (size_t)c->W * (size_t)c->H;
+    const bool cplx = !c->d.imag_time;
+    double *tmp = nullptr;
+    TSMI_HIP(hipMalloc(&tmp, (cplx ? 2 : 1) * n * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(tmp, e_re, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && cplx) e = hipMemcpyAsync(tmp + n, e_im, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    int st = TSMI_OK;
+    if (e == hipSuccess) st = launch_import_exp(c, tmp, cplx ? tmp + n : nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(TSMI_E_HIP, std::string("exp potential upload: ") + hipGetErrorString(e));
     if (st) return st;
     c->has_e = true;
     return TSMI_OK;
@@ -505,6 +552,12 @@ int tsmi_evolve(tsmi_ctx *c, int32_t n) {
         if (st) return st;
     }
     return TSMI_OK;
+}
+
+int tsmi_step(tsmi_ctx *c, int32_t exchange) {
+    if (!c) return fail(TSMI_E_INVALID, "tsmi_step: null context");
+    if (!c->has_state || !c->has_e) return fail(TSMI_E_INVALID, "tsmi_step: state and exp potential must be set");
+    return step_once(c, exchange ? 1 : 0);
 }
 
 int tsmi_squared_norm(tsmi_ctx *c, double *out) {

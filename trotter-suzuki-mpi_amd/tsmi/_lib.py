@@ -19,7 +19,7 @@ TSMI_OK = 0
 EXPORTS = (
     "tsmi_create", "tsmi_destroy", "tsmi_set_stream", "tsmi_set_state", "tsmi_set_state_device",
     "tsmi_set_exp_potential", "tsmi_set_exp_potential_device", "tsmi_set_potential",
-    "tsmi_set_potential_device", "tsmi_evolve", "tsmi_squared_norm", "tsmi_energy", "tsmi_expected_values",
+    "tsmi_set_potential_device", "tsmi_evolve", "tsmi_step", "tsmi_squared_norm", "tsmi_energy", "tsmi_expected_values",
     "tsmi_get_sample", "tsmi_get_state_device", "tsmi_synchronize", "tsmi_geometry", "tsmi_last_error",
 )
 
@@ -65,6 +65,7 @@ def load():
     L.tsmi_set_potential_device.argtypes = [C.c_void_p, C.c_void_p]
     L.tsmi_get_state_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.tsmi_evolve.argtypes = [C.c_void_p, C.c_int32]
+    L.tsmi_step.argtypes = [C.c_void_p, C.c_int32]
     L.tsmi_squared_norm.argtypes = [C.c_void_p, _dp]
     L.tsmi_energy.argtypes = [C.c_void_p, _dp]
     L.tsmi_expected_values.argtypes = [C.c_void_p, _dp]
