@@ -106,6 +106,55 @@ int tsmi_evolve(tsmi_ctx *ctx, int32_t n_steps);
  * wait_for_completion's renormalisation (:551-570).  Asynchronous. */
 int tsmi_step(tsmi_ctx *ctx, int32_t exchange);
 
+/* Gradient of a loss of the final state with respect to the potential.
+ * No reference counterpart (the reference evolves forward only).  Real time,
+ * closed axes and g = g_LHY = 0 only (TSMI_E_UNSUPPORTED otherwise).
+ *
+ * tsmi_evolve_checkpointed: tsmi_evolve(n_steps) that also keeps the state
+ * every `every` steps on the device (ceil(n_steps/every) copies of the tile).
+ * Asynchronous.  Any later evolve, step or set_* call drops the checkpoints.
+ *
+ * tsmi_adjoint / tsmi_adjoint_device: the reverse sweep over those steps.
+ * g_re/g_im = dL/dRe psi_N and dL/dIm psi_N (W*H FP64); out grad_v = dL/dV on
+ * every tile cell for E = exp(-i dt V) (W*H FP64); g0_re/g0_im (optional, may
+ * be NULL) = dL/dRe psi_0 and dL/dIm psi_0.  Each reverse step is the step at
+ * -dt applied to the adjoint and to the state together, the state re-anchored
+ * on the checkpoints; the FP64 gradient accumulates dt Im(conj(mu) psit) of the
+ * two mid-step states.  The host form syncs; the device form is asynchronous on
+ * the context stream.  The state of the context is left at psi_N. */
+int tsmi_evolve_checkpointed(tsmi_ctx *ctx, int32_t n_steps, int32_t every);
+int tsmi_adjoint(tsmi_ctx *ctx, const double *g_re, const double *g_im, double *grad_v,
+                 double *g0_re, double *g0_im);
+int tsmi_adjoint_device(tsmi_ctx *ctx, const double *g_re, const double *g_im, double *grad_v,
+                        double *g0_re, double *g0_im);
+
+/* Batches: many independent lattices advanced together.  No reference
+ * counterpart (the reference runs one Solver per lattice).  Each lattice has its
+ * own descriptor (size, boundaries, dt, mass, spacing, couplings); precision,
+ * time direction and whether g_LHY != 0 are common to the batch.  A step of the
+ * batch is one persistent launch over all lattices (plus one halo-exchange
+ * launch when some lattice is periodic) and gives each lattice exactly
+ * tsmi_evolve's step.
+ *
+ * tsmi_batch_evolve(n) records its launch sequence in a HIP graph the first time
+ * n is used from a given buffer parity and replays the graph afterwards;
+ * asynchronous.  tsmi_batch_tune times the batch step (`steps` steps per trial)
+ * for candidate segment heights, one size class (tile shape) at a time, keeps
+ * the fastest, restores the states and reports the tuned time per step.
+ * tsmi_batch_info: out[0..5] as tsmi_geometry for lattice i.
+ * set_state / set_exp_potential / get_state take W*H FP64 host arrays of
+ * lattice i; get_state syncs. */
+typedef struct tsmi_batch tsmi_batch;
+int tsmi_batch_create(const tsmi_desc *descs, int32_t n, tsmi_batch **out);
+void tsmi_batch_destroy(tsmi_batch *batch);
+int tsmi_batch_set_state(tsmi_batch *batch, int32_t i, const double *re, const double *im);
+int tsmi_batch_set_exp_potential(tsmi_batch *batch, int32_t i, const double *e_re, const double *e_im);
+int tsmi_batch_evolve(tsmi_batch *batch, int32_t n_steps);
+int tsmi_batch_get_state(tsmi_batch *batch, int32_t i, double *re, double *im);
+int tsmi_batch_tune(tsmi_batch *batch, int32_t steps, double *ms_per_step);
+int tsmi_batch_info(const tsmi_batch *batch, int32_t i, int32_t out[6]);
+int tsmi_batch_synchronize(tsmi_batch *batch);
+
 /* Squared norm of the inner region times dx*dy:
  * ITrotterKernel::calculate_squared_norm (src/cpukernel.cpp:526-549).  Syncs. */
 int tsmi_squared_norm(tsmi_ctx *ctx, double *out);

@@ -52,6 +52,9 @@ def lib():
         L.ts_oracle_energy.argtypes = [C.POINTER(Desc), _dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
                                        C.c_double, C.c_int, _dp]
         L.ts_oracle_expected_values.argtypes = [C.POINTER(Desc), _dp, _dp, _dp, _dp, _dp]
+        L.ts_oracle_adjoint.argtypes = [C.POINTER(Desc), C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double,
+                                        _dp, _dp, _dp, _dp, _dp]
+        L.ts_oracle_adjoint.restype = C.c_int
         _lib = L
     return _lib
 
@@ -131,6 +134,28 @@ def expected_values(wl, psi):
     out = np.zeros(10)
     lib().ts_oracle_expected_values(C.byref(d), _p(re), _p(im), _p(xc), _p(yc), _p(out))
     return out
+
+
+def adjoint(wl, g_final: np.ndarray, steps: int | None = None, every: int = 100):
+    """Oracle gradient of L(psi_N) with respect to V (FP64 discrete adjoint).
+
+    g_final = dL/dRe psi_N + i dL/dIm psi_N on the tile.  Returns
+    (dL/dV, dL/dRe psi_0 + i dL/dIm psi_0)."""
+    steps = wl.steps if steps is None else steps
+    d = make_desc(wl, norm_target=1.0)
+    er, ei = exp_potential(wl.v, wl.dt, wl.imag_time)
+    re = np.ascontiguousarray(wl.psi.real, dtype=np.float64)
+    im = np.ascontiguousarray(wl.psi.imag, dtype=np.float64)
+    gr = np.ascontiguousarray(g_final.real, dtype=np.float64)
+    gi = np.ascontiguousarray(g_final.imag, dtype=np.float64)
+    gv = np.zeros_like(re)
+    g0r = np.zeros_like(re)
+    g0i = np.zeros_like(re)
+    rc = lib().ts_oracle_adjoint(C.byref(d), steps, every, _p(re), _p(im), _p(er), _p(ei), wl.dt,
+                                 _p(gr), _p(gi), _p(gv), _p(g0r), _p(g0i))
+    if rc != 0:
+        raise ValueError(f"ts_oracle_adjoint: unsupported case (code {rc})")
+    return gv, g0r + 1j * g0i
 
 
 # ---------------------------------------------------------------------------

@@ -69,12 +69,16 @@ void ts_oracle_exp_potential(long long n, const double *v, double dt, int imag_t
     }
 }
 
+/* Tiles below this many cells run single-threaded: a parallel region costs more
+ * than the pass (and much more when the host is shared). */
+#define PAR_CELLS 65536
+
 /* Vertical pass V(s): cells (x,y),(x,y+1) pair when x+y+s is even, y+1 < H
  * (src/cpucartesian.cpp:4-44). */
 void ts_oracle_pass_vertical(const ts_oracle_desc *d, int s, double a, double b, double *re, double *im) {
     const size_t W = (size_t)d->W;
     const int H = d->H;
-    #pragma omp parallel for schedule(static)
+    #pragma omp parallel for schedule(static) if ((size_t)H * W >= PAR_CELLS)
     for (int y = 0; y < H - 1; ++y) {
         const size_t row = (size_t)y * W;
         for (size_t x = (size_t)((y + s) & 1); x < W; x += 2) {
@@ -89,7 +93,7 @@ void ts_oracle_pass_vertical(const ts_oracle_desc *d, int s, double a, double b,
 void ts_oracle_pass_horizontal(const ts_oracle_desc *d, int s, double a, double b, double *re, double *im) {
     const size_t W = (size_t)d->W;
     const int H = d->H;
-    #pragma omp parallel for schedule(static)
+    #pragma omp parallel for schedule(static) if ((size_t)H * W >= PAR_CELLS)
     for (int y = 0; y < H; ++y) {
         const size_t row = (size_t)y * W;
         for (size_t x = (size_t)((y + s) & 1); x + 1 < W; x += 2) {
@@ -105,7 +109,7 @@ void ts_oracle_pass_potential(const ts_oracle_desc *d, double *re, double *im,
                               const double *e_re, const double *e_im) {
     const size_t n = (size_t)d->W * (size_t)d->H;
     const double c = d->c_nl, cl = d->c_lhy;
-    #pragma omp parallel for schedule(static)
+    #pragma omp parallel for schedule(static) if (n >= PAR_CELLS)
     for (long long k = 0; k < (long long)n; ++k) {
         const double r = re[k], i = im[k];
         const double n2 = r * r + i * i;
@@ -309,4 +313,103 @@ void ts_oracle_expected_values(const ts_oracle_desc *d, const double *re, const 
     out[7] = -spy * ppy / sn;
     out[8] = -spypy * ppy * ppy / sn;
     out[9] = slz / sn;
+}
+
+/* ---------------------------------------------------------------------------
+ * Discrete adjoint of the real-time step with respect to the potential.
+ *
+ * Not a reference function: the reference has no gradient.  It is the exact
+ * derivative of ts_oracle_evolve (closed axes, real time, g = g_LHY = 0), so it
+ * is pinned by central finite differences of that forward (and therefore of the
+ * reference solver, which the forward matches bit for bit).
+ *
+ * With psi_{n+1} = A P A' psi_n, A' = the first four passes V0 H0 V1 H1, P the
+ * multiplication by E = exp(-i dt V) and A the last four, and the adjoint state
+ * G = dL/dRe psi + i dL/dIm psi:
+ *     mu_n = A^H G_{n+1},  dL/dV_j += dt Im(conj(mu_j) E_j chi_j),  chi_n = A' psi_n,
+ *     G_n = A'^H conj(E) mu_n.
+ * A real-time pass [[a, ib], [ib, a]] is symmetric, so its adjoint is the same
+ * pass with b -> -b; the passes of A^H run in the order of A' and vice versa.
+ * Forward states are kept every `every` steps and each segment is recomputed
+ * from its checkpoint before its reverse sweep.
+ * ------------------------------------------------------------------------- */
+#include <stdlib.h>
+#include <string.h>
+
+static void first_half(const ts_oracle_desc *d, double sgn, double *re, double *im) {
+    const int vert = d->H > 1;
+    if (vert) ts_oracle_pass_vertical(d, 0, d->aV, sgn * d->bV, re, im);
+    ts_oracle_pass_horizontal(d, 0, d->aH, sgn * d->bH, re, im);
+    if (vert) ts_oracle_pass_vertical(d, 1, d->aV, sgn * d->bV, re, im);
+    ts_oracle_pass_horizontal(d, 1, d->aH, sgn * d->bH, re, im);
+}
+
+static void second_half(const ts_oracle_desc *d, double sgn, double *re, double *im) {
+    const int vert = d->H > 1;
+    ts_oracle_pass_horizontal(d, 1, d->aH, sgn * d->bH, re, im);
+    if (vert) ts_oracle_pass_vertical(d, 1, d->aV, sgn * d->bV, re, im);
+    ts_oracle_pass_horizontal(d, 0, d->aH, sgn * d->bH, re, im);
+    if (vert) ts_oracle_pass_vertical(d, 0, d->aV, sgn * d->bV, re, im);
+}
+
+int ts_oracle_adjoint(const ts_oracle_desc *d, int n_steps, int every,
+                      const double *psi_re, const double *psi_im,
+                      const double *e_re, const double *e_im, double dt,
+                      const double *g_re, const double *g_im,
+                      double *grad_v, double *g0_re, double *g0_im) {
+    if (d->imag_time || d->periodic_x || d->periodic_y || d->c_nl != 0. || d->c_lhy != 0. ||
+        n_steps < 0 || every < 1)
+        return -1;
+    const size_t n = (size_t)d->W * (size_t)d->H;
+    const int n_ck = n_steps > 0 ? (n_steps - 1) / every + 1 : 0;
+    const int seg_len = n_steps < every ? n_steps : every;
+    double *ck = malloc(sizeof(double) * 2 * n * (size_t)(n_ck > 0 ? n_ck : 1));
+    double *chi = malloc(sizeof(double) * 2 * n * (size_t)(seg_len > 0 ? seg_len : 1));
+    double *st = malloc(sizeof(double) * 2 * n);
+    double *gg = malloc(sizeof(double) * 2 * n);
+    if (!ck || !chi || !st || !gg) {
+        free(ck); free(chi); free(st); free(gg);
+        return -2;
+    }
+    /* forward sweep, keeping psi_{k*every} */
+    memcpy(st, psi_re, sizeof(double) * n);
+    memcpy(st + n, psi_im, sizeof(double) * n);
+    for (int i = 0; i < n_steps; ++i) {
+        if (i % every == 0) memcpy(ck + 2 * n * (size_t)(i / every), st, sizeof(double) * 2 * n);
+        ts_oracle_full_step(d, st, st + n, e_re, e_im);
+    }
+    memcpy(gg, g_re, sizeof(double) * n);
+    memcpy(gg + n, g_im, sizeof(double) * n);
+    for (size_t k = 0; k < n; ++k) grad_v[k] = 0.;
+    double *gr = gg, *gi = gg + n;
+    for (int seg = n_ck - 1; seg >= 0; --seg) {
+        const int s0 = seg * every;
+        const int s1 = s0 + every < n_steps ? s0 + every : n_steps;
+        /* recompute chi_n = A' psi_n over the segment */
+        memcpy(st, ck + 2 * n * (size_t)seg, sizeof(double) * 2 * n);
+        for (int i = s0; i < s1; ++i) {
+            first_half(d, 1., st, st + n);
+            memcpy(chi + 2 * n * (size_t)(i - s0), st, sizeof(double) * 2 * n);
+            ts_oracle_pass_potential(d, st, st + n, e_re, e_im);
+            second_half(d, 1., st, st + n);
+        }
+        for (int i = s1 - 1; i >= s0; --i) {
+            const double *cr = chi + 2 * n * (size_t)(i - s0), *ci = cr + n;
+            first_half(d, -1., gr, gi);                      /* mu = A^H G */
+            for (size_t k = 0; k < n; ++k) {
+                const double er = e_re[k], ei = e_im[k];
+                const double pr = er * cr[k] - ei * ci[k];   /* E chi */
+                const double pi = er * ci[k] + ei * cr[k];
+                grad_v[k] += dt * (gr[k] * pi - gi[k] * pr);
+                const double r = gr[k], m = gi[k];           /* conj(E) mu */
+                gr[k] = er * r + ei * m;
+                gi[k] = er * m - ei * r;
+            }
+            second_half(d, -1., gr, gi);                     /* A'^H */
+        }
+    }
+    if (g0_re) memcpy(g0_re, gr, sizeof(double) * n);
+    if (g0_im) memcpy(g0_im, gi, sizeof(double) * n);
+    free(ck); free(chi); free(st); free(gg);
+    return 0;
 }

@@ -88,6 +88,20 @@ void ts_oracle_energy(const ts_oracle_desc *d, const double *re, const double *i
 void ts_oracle_expected_values(const ts_oracle_desc *d, const double *re, const double *im,
                                const double *x_coord, const double *y_coord, double out[10]);
 
+/* Gradient of a loss L(psi_N) with respect to the potential V on every tile
+ * cell, through n_steps real-time steps of ts_oracle_evolve (closed axes,
+ * g = g_LHY = 0): the discrete adjoint, not a reference function.
+ * g_re/g_im = dL/dRe psi_N, dL/dIm psi_N; E = exp(-i dt V) as for the forward.
+ * Out: grad_v[W*H] = dL/dV; g0_re/g0_im (may be NULL) = dL/dRe psi_0, dL/dIm psi_0.
+ * Checkpoints every `every` steps, each segment recomputed forward.
+ * Returns 0, -1 for an unsupported descriptor, -2 when out of memory.
+ * Pinned: central finite differences of ts_oracle_evolve (tests/test_adjoint.py). */
+int ts_oracle_adjoint(const ts_oracle_desc *d, int n_steps, int every,
+                      const double *psi_re, const double *psi_im,
+                      const double *e_re, const double *e_im, double dt,
+                      const double *g_re, const double *g_im,
+                      double *grad_v, double *g0_re, double *g0_im);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,126 @@
+"""The HIP reverse sweep (tsmi_evolve_checkpointed + tsmi_adjoint) against the
+oracle's FP64 discrete adjoint, which is pinned by central finite differences
+(tests/test_adjoint_oracle.py), and the torch.autograd bridge.
+
+Bars: FP64 within 1e-10 of the oracle (max-norm, relative to the largest
+gradient entry); FP32 within 1e-4 (the FP32 adjoint target)."""
+import numpy as np
+import pytest
+
+from tsmi import workloads as W
+from tsmi.losses import infidelity
+
+pytestmark = pytest.mark.gpu
+
+
+def case(n_x, n_y, steps, dt=2e-3, length=12.0, seed=W.SEED_BASE + 91):
+    wl = W.small_case(n_x, n_y, False, False, steps=steps, dt=dt, length=length, seed=seed)
+    lat = wl.lattice
+    tx, ty = W.uniform(seed + 1, 2, lo=-1.5, hi=1.5)
+    target = W.normalise(lat, W.gaussian_packet(lat, tx, ty, 0.0, 0.0, width=1.5))
+    return wl, target
+
+
+def rel_max(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def hip_adjoint(wl, target, oracle, precision, every, tile_rows=0):
+    from tsmi.hip import kernel_for
+    e = oracle.exp_potential(wl.v, wl.dt, False)
+    k = kernel_for(wl, precision=precision, tile_rows=tile_rows, e_planes=e)
+    try:
+        k.evolve_checkpointed(wl.steps, every)
+        psi = k.sample()
+        L, g = infidelity(wl.lattice, target, psi)
+        gv, g0 = k.adjoint(g)
+        # the context still holds psi_N
+        assert np.array_equal(k.sample(), psi)
+        return L, gv, g0
+    finally:
+        k.close()
+
+
+CASES = [
+    dict(n_x=96, n_y=72, steps=120, every=25),
+    dict(n_x=601, n_y=75, steps=40, every=7),       # several strips, ragged edges
+    dict(n_x=37, n_y=1, steps=30, every=30),        # single row: no vertical passes
+    dict(n_x=64, n_y=64, steps=50, every=1),        # a checkpoint at every step
+]
+
+
+@pytest.mark.parametrize("c", CASES, ids=lambda c: f"{c['n_x']}x{c['n_y']}_s{c['steps']}_k{c['every']}")
+def test_adjoint_fp64_matches_oracle(c, oracle):
+    wl, target = case(c["n_x"], c["n_y"], c["steps"])
+    _, g = infidelity(wl.lattice, target, oracle.evolve(wl))
+    ref_v, ref_0 = oracle.adjoint(wl, g, every=10)
+    L, gv, g0 = hip_adjoint(wl, target, oracle, 64, c["every"])
+    assert rel_max(gv, ref_v) < 1e-10
+    assert rel_max(g0, ref_0) < 1e-10
+
+
+@pytest.mark.parametrize("c", CASES[:2], ids=lambda c: f"{c['n_x']}x{c['n_y']}")
+def test_adjoint_fp32_within_tolerance(c, oracle):
+    wl, target = case(c["n_x"], c["n_y"], c["steps"])
+    _, g = infidelity(wl.lattice, target, oracle.evolve(wl))
+    ref_v, ref_0 = oracle.adjoint(wl, g, every=10)
+    _, gv, g0 = hip_adjoint(wl, target, oracle, 32, c["every"])
+    assert rel_max(gv, ref_v) < 1e-4
+    assert rel_max(g0, ref_0) < 1e-4
+
+
+def test_adjoint_segment_heights_agree(oracle):
+    wl, target = case(120, 90, 30)
+    _, a, _ = hip_adjoint(wl, target, oracle, 64, 10, tile_rows=8)
+    _, b, _ = hip_adjoint(wl, target, oracle, 64, 10, tile_rows=30)
+    assert rel_max(a, b) < 1e-12
+
+
+def test_adjoint_errors():
+    from tsmi import _lib
+    from tsmi.hip import kernel_for
+    wl = W.small_case(32, 32, True, False, steps=3)
+    k = kernel_for(wl)
+    try:
+        with pytest.raises(_lib.TsmiError, match="closed"):
+            k.evolve_checkpointed(3, 1)
+    finally:
+        k.close()
+    wl = W.small_case(32, 32, False, False, steps=3)
+    k = kernel_for(wl)
+    try:
+        with pytest.raises(_lib.TsmiError, match="checkpointed"):
+            k.adjoint(np.zeros_like(wl.psi))
+        k.evolve_checkpointed(3, 2)
+        k.evolve(1)                      # drops the checkpoints
+        with pytest.raises(_lib.TsmiError, match="checkpointed"):
+            k.adjoint(np.zeros_like(wl.psi))
+    finally:
+        k.close()
+
+
+def test_torch_autograd_matches_oracle(oracle):
+    import torch
+    from tsmi import autograd as TA
+    from tsmi.hip import HipKernel
+    wl, target = case(80, 64, 60)
+    lat = wl.lattice
+    dev = torch.device("cuda:0")
+    k = HipKernel(lat, delta_t=wl.dt, precision=64)
+    try:
+        v = torch.tensor(wl.v, device=dev, requires_grad=True)
+        p_re = torch.tensor(np.ascontiguousarray(wl.psi.real), device=dev, requires_grad=True)
+        p_im = torch.tensor(np.ascontiguousarray(wl.psi.imag), device=dev, requires_grad=True)
+        t_re = torch.tensor(np.ascontiguousarray(target.real), device=dev)
+        t_im = torch.tensor(np.ascontiguousarray(target.imag), device=dev)
+        re, im = TA.evolve(k, v, p_re, p_im, wl.steps, every=16)
+        loss = TA.infidelity(t_re, t_im, re, im, lat.delta_x * lat.delta_y)
+        loss.backward()
+        torch.cuda.synchronize()
+        L_ref, g = infidelity(lat, target, oracle.evolve(wl))
+        ref_v, ref_0 = oracle.adjoint(wl, g, every=10)
+        assert abs(float(loss) - L_ref) < 1e-12
+        assert rel_max(v.grad.cpu().numpy(), ref_v) < 1e-10
+        assert rel_max(p_re.grad.cpu().numpy() + 1j * p_im.grad.cpu().numpy(), ref_0) < 1e-10
+    finally:
+        k.close()

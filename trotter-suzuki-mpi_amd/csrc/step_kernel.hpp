@@ -436,20 +436,21 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
         TV x;
 #pragma unroll
         for (int k = 0; k < E; ++k) x[k] = v[i * E + k];
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, voff + 16 * i, soff, 0);
+        // the row offset goes into the vector offset with a zero scalar offset:
+        // a 128-bit store then carries a one-wait-state hazard on its data
+        // registers that the compiler models (it does not for a store with a
+        // scalar-register offset)
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, voff + soff + 16 * i, 0, 0);
     }
 }
 
 // ---------------------------------------------------------------------------
 // the kernel
 // ---------------------------------------------------------------------------
+// One wavefront's share of a step: strip (wave mod n_strips), segment
+// (wave / n_strips).  `wave` is wave-uniform.
 template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
-__global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
-    const int lane = threadIdx.x & (kWave - 1);
-    // wave-uniform by construction; readfirstlane lets the compiler keep every
-    // quantity derived from it (segment, strip, row bounds) in scalar registers
-    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
-    if (wave >= p.n_waves) return;
+__device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave, const int lane) {
     constexpr int SW = kWave * VEC - 2 * kApron;       // output columns per strip
     const int strip = wave % p.n_strips;
     const int seg = wave / p.n_strips;
@@ -666,6 +667,16 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
     }
 }
 
+template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
+__global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
+    const int lane = threadIdx.x & (kWave - 1);
+    // wave-uniform by construction; readfirstlane lets the compiler keep every
+    // quantity derived from it (segment, strip, row bounds) in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
+    if (wave >= p.n_waves) return;
+    step_wave<T, IMAG, VEC, LHY, MODE>(p, wave, lane);
+}
+
 // ---------------------------------------------------------------------------
 // periodic halo exchange
 // ---------------------------------------------------------------------------
@@ -676,37 +687,47 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
 // source directly (a y-halo cell whose source row had its x halo refreshed
 // takes the inner cell both copies lead to), so sources are never destinations
 // and one launch suffices.
+// Destinations: [0, hy) and [H-hy, H) full rows (periodic y), then the x bands
+// of the rows in between (periodic x).
+__device__ __forceinline__ int exchange_cells(int W, int H, int px, int py, int hx, int hy) {
+    return (py ? 2 * hy * W : 0) + (px ? 2 * hx * (H - (py ? 2 * hy : 0)) : 0);
+}
+
+// Refreshes destination cell i (0 <= i < exchange_cells(...)).
+template <typename T>
+__device__ __forceinline__ void exchange_cell(int i, T *re, T *im, int W, int H, int pitch, int px, int py,
+                                              int hx, int hy) {
+    const int nx = W - 2 * hx, ny = H - 2 * hy;
+    const int rx0 = py ? hy : 0, rx1 = rx0 + max(0, H - 2 * hy);
+    const int n_yband = py ? 2 * hy * W : 0;
+    int x, y;
+    if (i < n_yband) {
+        const int r = i / W;
+        x = i - r * W;
+        y = r < hy ? r : H - 2 * hy + r;
+    } else {
+        const int j = i - n_yband;
+        const int r = j / (2 * hx);
+        const int c = j - r * 2 * hx;
+        y = (py ? hy : 0) + r;
+        x = c < hx ? c : W - 2 * hx + c;
+    }
+    int sy = y, sx = x;
+    if (py && (y < hy || y >= H - hy)) sy = y < hy ? y + ny : y - ny;
+    const bool xd = px && (x < hx || x >= W - hx) && sy >= rx0 && sy < rx1;
+    if (xd) sx = x < hx ? x + nx : x - nx;
+    if (sx == x && sy == y) return;
+    const long d = (long)y * pitch + x, s = (long)sy * pitch + sx;
+    re[d] = re[s];
+    im[d] = im[s];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) halo_exchange_kernel(T *re, T *im, int W, int H, int pitch, int px, int py,
                                                             int hx, int hy) {
-    const int nx = W - 2 * hx, ny = H - 2 * hy;
-    const int rx0 = py ? hy : 0, rx1 = rx0 + max(0, H - 2 * hy);
-    // destinations: [0,hy) and [H-hy,H) full rows (py); x bands of other rows (px)
-    const int n_yband = py ? 2 * hy * W : 0;
-    const int n_xband = px ? 2 * hx * (H - (py ? 2 * hy : 0)) : 0;
-    const int n = n_yband + n_xband;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int x, y;
-        if (i < n_yband) {
-            const int r = i / W;
-            x = i - r * W;
-            y = r < hy ? r : H - 2 * hy + r;
-        } else {
-            const int j = i - n_yband;
-            const int r = j / (2 * hx);
-            const int c = j - r * 2 * hx;
-            y = (py ? hy : 0) + r;
-            x = c < hx ? c : W - 2 * hx + c;
-        }
-        int sy = y, sx = x;
-        if (py && (y < hy || y >= H - hy)) sy = y < hy ? y + ny : y - ny;
-        const bool xd = px && (x < hx || x >= W - hx) && sy >= rx0 && sy < rx1;
-        if (xd) sx = x < hx ? x + nx : x - nx;
-        if (sx == x && sy == y) continue;
-        const long d = (long)y * pitch + x, s = (long)sy * pitch + sx;
-        re[d] = re[s];
-        im[d] = im[s];
-    }
+    const int n = exchange_cells(W, H, px, py, hx, hy);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        exchange_cell<T>(i, re, im, W, H, pitch, px, py, hx, hy);
 }
 
 }  // namespace tsmi

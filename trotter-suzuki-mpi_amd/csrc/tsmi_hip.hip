@@ -17,6 +17,9 @@
 #include <string>
 #include <vector>
 
+#include "adjoint_kernel.hpp"
+#include "common.hpp"
+#include "frame.hpp"
 #include "observables.hpp"
 #include "step_kernel.hpp"
 
@@ -30,6 +33,44 @@ int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+
+}  // namespace
+
+namespace tsmi {
+
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+int check_desc(const tsmi_desc &d, const char *who) {
+    const std::string w(who);
+    if (d.dim_x < 1 || d.dim_y < 1) return fail(TSMI_E_INVALID, w + ": lattice dimensions must be positive");
+    if (d.precision != 32 && d.precision != 64) return fail(TSMI_E_INVALID, w + ": precision must be 32 or 64");
+    if (d.halo_x != 4 || d.halo_y != 4)
+        return fail(TSMI_E_UNSUPPORTED, w + ": only the halo width 4 of a non-rotating lattice is supported");
+    if (!(d.delta_x > 0) || !(d.delta_y > 0) || !(d.mass != 0))
+        return fail(TSMI_E_INVALID, w + ": lattice spacing and mass must be non-zero");
+    if ((d.periodic_x && d.dim_x < 2 * d.halo_x) || (d.periodic_y && d.dim_y < 2 * d.halo_y))
+        return fail(TSMI_E_INVALID, w + ": a periodic axis needs at least 2*halo points");
+    if (d.tile_rows < 0) return fail(TSMI_E_INVALID, w + ": tile_rows must be >= 0");
+    const long long W = (long long)d.dim_x + (d.periodic_x ? 2 * d.halo_x : 0);
+    const long long H = (long long)d.dim_y + (d.periodic_y ? 2 * d.halo_y : 0);
+    const long long pitch = (W + 63) / 64 * 64;
+    // a wave addresses its segment with 32-bit byte offsets: one FP64 row of
+    // (apron + 2) rows must fit well inside 2^31 bytes
+    if (pitch * 8 * 16 >= (1LL << 31) || H > (1 << 30))
+        return fail(TSMI_E_INVALID, w + ": lattice too large");
+    return TSMI_OK;
+}
+
+// Largest even segment height whose rows plus the 4-row aprons keep a wave's
+// byte offsets below 2^31 (FP64 gradient planes included).
+int max_seg_rows(long long pitch) {
+    const long long rows = ((1LL << 31) - 1) / (pitch * 8) - 2 * kApron;
+    return (int)std::max<long long>(2, std::min<long long>(rows, 1 << 30) / 2 * 2);
+}
+
+}  // namespace tsmi
+
+namespace {
 
 #define TSMI_HIP(call)                                                                       \
     do {                                                                                     \
@@ -64,6 +105,16 @@ struct tsmi_ctx {
     double c_nl_s = 0, c_lhy_s = 0;        // couplings rescaled to the mid-step frame
     int mode = 0;                          // 0 / 1: both axes in form A / B; -1: mixed
     int ix0 = 0, ix1 = 0, iy0 = 0, iy1 = 0;
+    // reverse sweep (real time, closed axes, no nonlinearity)
+    bool adj_ok = false;
+    double amH = 1, atH = 0, amV = 1, atV = 0, aunH = 1, aunV = 1;   // frame of the step at -dt
+    double corr_abs = 1.0;                                         // |corr| = |kappa|^2
+    void *ck = nullptr;               // checkpoints: 2 planes each
+    size_t ck_cap = 0;
+    int ck_every = 0, ck_steps = 0, ck_count = 0;
+    bool ck_valid = false;
+    void *adj = nullptr;              // 8 planes: adjoint re/im x2, reversed state re/im x2
+    double *grad = nullptr;           // FP64, pitched
 };
 
 namespace {
@@ -87,7 +138,8 @@ int free_all(tsmi_ctx *c) {
         if (c->psi_re[b]) (void)hipFree(c->psi_re[b]);
         if (c->psi_im[b]) (void)hipFree(c->psi_im[b]);
     }
-    void *ptrs[] = {c->e_re, c->e_im, c->v, c->xc, c->yc, c->div, c->partials, c->counter, c->obs_partials};
+    void *ptrs[] = {c->e_re, c->e_im, c->v, c->xc, c->yc, c->div, c->partials, c->counter, c->obs_partials,
+                    c->ck, c->adj, c->grad};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->own) (void)hipStreamDestroy(c->own);
@@ -263,6 +315,86 @@ int step_once(tsmi_ctx *c, int exchange) {
     return c->esz == 4 ? step_typed<float, 4>(c, exchange) : step_typed<double, 2>(c, exchange);
 }
 
+// One reverse step: (s_in, g_in) -> (s_out, g_out), gradient into c->grad.
+template <typename T, int VEC>
+int launch_adjoint(tsmi_ctx *c, const void *s_re, const void *s_im, const void *g_re, const void *g_im,
+                   void *so_re, void *so_im, void *go_re, void *go_im) {
+    AdjParams<T> p;
+    std::memset(&p, 0, sizeof(p));
+    p.s_in_re = (const T *)s_re; p.s_in_im = (const T *)s_im;
+    p.g_in_re = (const T *)g_re; p.g_in_im = (const T *)g_im;
+    p.s_out_re = (T *)so_re; p.s_out_im = (T *)so_im;
+    p.g_out_re = (T *)go_re; p.g_out_im = (T *)go_im;
+    p.e_re = (const T *)c->e_re; p.e_im = (const T *)c->e_im;
+    p.grad = c->grad;
+    p.W = c->W; p.H = c->H; p.pitch = c->pitch;
+    p.n_strips = c->n_strips; p.n_waves = c->n_waves; p.seg_rows = c->seg_rows;
+    p.mH = (T)c->amH; p.tH = (T)c->atH; p.mV = (T)c->amV; p.tV = (T)c->atV;
+    p.unH = (T)c->aunH; p.unV = (T)c->aunV;
+    p.gscale = c->d.delta_t * c->corr_abs;
+    const int blocks = (c->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL((adjoint_kernel<T, VEC>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
+    TSMI_HIP(hipGetLastError());
+    return TSMI_OK;
+}
+
+int adjoint_step(tsmi_ctx *c, const void *s_re, const void *s_im, const void *g_re, const void *g_im,
+                 void *so_re, void *so_im, void *go_re, void *go_im) {
+    return c->esz == 4 ? launch_adjoint<float, 4>(c, s_re, s_im, g_re, g_im, so_re, so_im, go_re, go_im)
+                       : launch_adjoint<double, 2>(c, s_re, s_im, g_re, g_im, so_re, so_im, go_re, go_im);
+}
+
+// The reverse sweep over the steps of the last checkpointed evolve.  g_re/g_im
+// are FP64 W x H planes on the device; the gradient is left in c->grad and the
+// adjoint of the initial state in the returned planes.
+int reverse_sweep(tsmi_ctx *c, const double *g_re, const double *g_im, void **g0_re, void **g0_im) {
+    const size_t pb = plane_bytes(c);
+    const size_t pd = (size_t)c->pitch * (size_t)c->H * sizeof(double);
+    if (!c->adj) TSMI_HIP(hipMalloc(&c->adj, 8 * pb));
+    if (!c->grad) TSMI_HIP(hipMalloc((void **)&c->grad, pd));
+    char *a = (char *)c->adj;
+    void *L[2][2] = {{a, a + pb}, {a + 2 * pb, a + 3 * pb}};
+    void *S[2][2] = {{a + 4 * pb, a + 5 * pb}, {a + 6 * pb, a + 7 * pb}};
+    // rows are padded; the padding of every plane must read as zero
+    TSMI_HIP(hipMemsetAsync(c->adj, 0, 8 * pb, c->stream));
+    TSMI_HIP(hipMemsetAsync(c->grad, 0, pd, c->stream));
+    int st = import_device(c, L[0][0], g_re);
+    if (!st) st = import_device(c, L[0][1], g_im);
+    if (st) return st;
+    const void *s_re = c->psi_re[c->cur], *s_im = c->psi_im[c->cur];
+    int k = 0;
+    for (int n = c->ck_steps - 1; n >= 0; --n) {
+        const int o = 1 - k;
+        st = adjoint_step(c, s_re, s_im, L[k][0], L[k][1], S[o][0], S[o][1], L[o][0], L[o][1]);
+        if (st) return st;
+        if (n % c->ck_every == 0) {   // psi_n is a checkpoint: use it instead of the reversed state
+            const char *ck = (const char *)c->ck + (size_t)(n / c->ck_every) * 2 * pb;
+            s_re = ck;
+            s_im = ck + pb;
+        } else {
+            s_re = S[o][0];
+            s_im = S[o][1];
+        }
+        k = o;
+    }
+    *g0_re = L[k][0];
+    *g0_im = L[k][1];
+    return TSMI_OK;
+}
+
+// pitched plane of the context precision -> W x H FP64 on the device
+int export_device(tsmi_ctx *c, const void *src, double *dst) {
+    const int blocks = 2048;
+    if (c->esz == 8)
+        hipLaunchKernelGGL(export_plane<double>, dim3(blocks), dim3(256), 0, c->stream, (const double *)src, dst,
+                           c->W, c->H, c->pitch, (const double *)nullptr);
+    else
+        hipLaunchKernelGGL(export_plane<float>, dim3(blocks), dim3(256), 0, c->stream, (const float *)src, dst,
+                           c->W, c->H, c->pitch, (const double *)nullptr);
+    TSMI_HIP(hipGetLastError());
+    return TSMI_OK;
+}
+
 template <typename T>
 int run_observables(tsmi_ctx *c, bool energy_margin, double out[OBS_COUNT]) {
     double div = 1.0;
@@ -318,19 +450,10 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     if (!desc || !out) return fail(TSMI_E_INVALID, "tsmi_create: null argument");
     *out = nullptr;
     const tsmi_desc &d = *desc;
-    if (d.dim_x < 1 || d.dim_y < 1) return fail(TSMI_E_INVALID, "tsmi_create: lattice dimensions must be positive");
-    if (d.precision != 32 && d.precision != 64) return fail(TSMI_E_INVALID, "tsmi_create: precision must be 32 or 64");
-    if (d.halo_x != 4 || d.halo_y != 4)
-        return fail(TSMI_E_UNSUPPORTED, "tsmi_create: only the halo width 4 of a non-rotating lattice is supported");
-    if (!(d.delta_x > 0) || !(d.delta_y > 0) || !(d.mass != 0))
-        return fail(TSMI_E_INVALID, "tsmi_create: lattice spacing and mass must be non-zero");
-    if ((d.periodic_x && d.dim_x < 2 * d.halo_x) || (d.periodic_y && d.dim_y < 2 * d.halo_y))
-        return fail(TSMI_E_INVALID, "tsmi_create: a periodic axis needs at least 2*halo points");
+    if (int st = check_desc(d, "tsmi_create")) return st;
     const long long W = (long long)d.dim_x + (d.periodic_x ? 2 * d.halo_x : 0);
     const long long H = (long long)d.dim_y + (d.periodic_y ? 2 * d.halo_y : 0);
     const long long pitch = (W + 63) / 64 * 64;
-    if (pitch * H >= (1LL << 40) || W > (1 << 30) || H > (1 << 30))
-        return fail(TSMI_E_INVALID, "tsmi_create: lattice too large");
 
     tsmi_ctx *c = new tsmi_ctx();
     c->d = d;
@@ -355,6 +478,7 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
             seg = (int)std::max<long>(16, (H + segs - 1) / segs);
         }
     }
+    seg = std::min(seg, max_seg_rows(pitch));
     seg = (seg + 1) / 2 * 2;   // segments start on even rows (the kernel's pair patterns assume it)
     c->seg_rows = std::min<int>(seg, c->H);
     c->n_segs = (c->H + c->seg_rows - 1) / c->seg_rows;
@@ -362,52 +486,20 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     c->ix0 = d.periodic_x ? d.halo_x : 0; c->ix1 = c->W - c->ix0;
     c->iy0 = d.periodic_y ? d.halo_y : 0; c->iy1 = c->H - c->iy0;
 
-    // kinetic and nonlinear coefficients in FP64 (src/cpukernel.cpp:195-216)
-    const double bx = d.delta_t / (4. * d.mass * d.delta_x * d.delta_x);
-    const double by = d.delta_t / (4. * d.mass * d.delta_y * d.delta_y);
-    if (d.imag_time) {
-        c->aH = std::cosh(bx); c->bH = std::sinh(bx);
-        c->aV = std::cosh(by); c->bV = std::sinh(by);
-    } else {
-        c->aH = std::cos(bx); c->bH = std::sin(bx);
-        c->aV = std::cos(by); c->bV = std::sin(by);
-    }
-    c->c_nl = d.coupling * d.delta_t;
-    c->c_lhy = d.coupling_lhy * d.delta_t;
-    // Scaled-frame factors per axis (the kernel's fpair): form A (m, t) = (1, b/a)
-    // when |a| >= |b| or in imaginary time, form B (a/b, 1) otherwise, both
-    // rounded to the kernel precision.  g = lambda / lambda~ compares the exact
-    // smooth-mode eigenvalue of a pass (a + ib, or a + b in imaginary time) with
-    // the computed one (m + it, or m + t); its modulus is exact for every mode.
-    // A step runs 4 passes per axis, 2 of them before the potential step, so
-    // E takes corr = gH^4 gV^4 and the couplings see |psi|^2 / |gH gV|^4.
-    struct Axis { double m, t, un; std::complex<double> g; int form; };
-    auto axis = [&](double a, double b) {
-        Axis x;
-        x.form = (d.imag_time || std::fabs(a) >= std::fabs(b)) ? 0 : 1;
-        double m = x.form == 0 ? 1.0 : a / b, t = x.form == 0 ? b / a : 1.0;
-        if (c->esz == 4) { m = (float)m; t = (float)t; }
-        x.m = m; x.t = t;
-        const std::complex<double> lam = d.imag_time ? std::complex<double>(a + b, 0.0) : std::complex<double>(a, b);
-        const std::complex<double> lt = d.imag_time ? std::complex<double>(m + t, 0.0) : std::complex<double>(m, t);
-        x.g = lam / lt;
-        double un = 1.0 / x.g.real();
-        if (c->esz == 4) un = (float)un;
-        x.un = un;
-        return x;
-    };
-    const Axis ah = axis(c->aH, c->bH);
-    const Axis av = c->H > 1 ? axis(c->aV, c->bV) : Axis{1.0, 0.0, 1.0, std::complex<double>(1.0, 0.0), ah.form};
-    c->mH = ah.m; c->tH = ah.t; c->unH = ah.un;
-    c->mV = av.m; c->tV = av.t; c->unV = av.un;
-    c->mode = ah.form == av.form ? ah.form : -1;
-    std::complex<double> corr = ah.g * ah.g * ah.g * ah.g;
-    if (c->H > 1) corr *= av.g * av.g * av.g * av.g;
-    c->corr_re = corr.real();
-    c->corr_im = d.imag_time ? 0.0 : corr.imag();
-    const double sn = std::pow(std::abs(ah.g) * std::abs(av.g), 4);
-    c->c_nl_s = c->c_nl * sn;
-    c->c_lhy_s = c->c_lhy * std::pow(sn, 1.5);
+    // kinetic and nonlinear coefficients in FP64 (src/cpukernel.cpp:195-216) and
+    // the kernel's scaled-frame factors
+    const StepCoefs k = step_coefs(d, c->H, c->esz);
+    c->aH = k.aH; c->bH = k.bH; c->aV = k.aV; c->bV = k.bV;
+    c->c_nl = k.c_nl; c->c_lhy = k.c_lhy;
+    c->mH = k.mH; c->tH = k.tH; c->unH = k.unH;
+    c->mV = k.mV; c->tV = k.tV; c->unV = k.unV;
+    c->mode = k.mode;
+    c->corr_re = k.corr_re; c->corr_im = k.corr_im;
+    c->c_nl_s = k.c_nl_s; c->c_lhy_s = k.c_lhy_s;
+    c->adj_ok = !d.imag_time && !d.periodic_x && !d.periodic_y && d.coupling == 0.0 && d.coupling_lhy == 0.0;
+    c->amH = k.amH; c->atH = k.atH; c->aunH = k.aunH;
+    c->amV = k.amV; c->atV = k.atV; c->aunV = k.aunV;
+    c->corr_abs = k.corr_abs;
 
     auto bail = [&](int code, const std::string &m) {
         free_all(c);
@@ -459,6 +551,7 @@ int tsmi_set_stream(tsmi_ctx *c, void *s) {
 }
 
 int tsmi_set_state(tsmi_ctx *c, const double *re, const double *im) {
+    if (c) c->ck_valid = false;
     if (!c || !re || !im) return fail(TSMI_E_INVALID, "tsmi_set_state: null argument");
     int st = upload_plane(c, c->psi_re[c->cur], re);
     if (!st) st = upload_plane(c, c->psi_im[c->cur], im);
@@ -469,6 +562,7 @@ int tsmi_set_state(tsmi_ctx *c, const double *re, const double *im) {
 }
 
 int tsmi_set_state_device(tsmi_ctx *c, const double *re, const double *im) {
+    if (c) c->ck_valid = false;
     if (!c || !re || !im) return fail(TSMI_E_INVALID, "tsmi_set_state_device: null argument");
     int st = import_device(c, c->psi_re[c->cur], re);
     if (!st) st = import_device(c, c->psi_im[c->cur], im);
@@ -479,6 +573,7 @@ int tsmi_set_state_device(tsmi_ctx *c, const double *re, const double *im) {
 }
 
 int tsmi_set_exp_potential_device(tsmi_ctx *c, const double *e_re, const double *e_im) {
+    if (c) c->ck_valid = false;
     if (!c || !e_re) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential_device: null argument");
     if (!c->d.imag_time && !e_im) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential_device: real time needs Im(E)");
     const int st = launch_import_exp(c, e_re, c->d.imag_time ? nullptr : e_im);
@@ -515,6 +610,7 @@ int tsmi_get_state_device(tsmi_ctx *c, double *re, double *im) {
 }
 
 int tsmi_set_exp_potential(tsmi_ctx *c, const double *e_re, const double *e_im) {
+    if (c) c->ck_valid = false;
     if (!c || !e_re) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential: null argument");
     if (!c->d.imag_time && !e_im) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential: real time needs Im(E)");
     // stage the FP64 table on the device; the import kernel applies corr and rounds
@@ -544,6 +640,7 @@ int tsmi_set_potential(tsmi_ctx *c, const double *v) {
 }
 
 int tsmi_evolve(tsmi_ctx *c, int32_t n) {
+    if (c) c->ck_valid = false;
     if (!c) return fail(TSMI_E_INVALID, "tsmi_evolve: null context");
     if (n < 0) return fail(TSMI_E_INVALID, "tsmi_evolve: negative step count");
     if (!c->has_state || !c->has_e) return fail(TSMI_E_INVALID, "tsmi_evolve: state and exp potential must be set");
@@ -555,9 +652,81 @@ int tsmi_evolve(tsmi_ctx *c, int32_t n) {
 }
 
 int tsmi_step(tsmi_ctx *c, int32_t exchange) {
+    if (c) c->ck_valid = false;
     if (!c) return fail(TSMI_E_INVALID, "tsmi_step: null context");
     if (!c->has_state || !c->has_e) return fail(TSMI_E_INVALID, "tsmi_step: state and exp potential must be set");
     return step_once(c, exchange ? 1 : 0);
+}
+
+int tsmi_evolve_checkpointed(tsmi_ctx *c, int32_t n, int32_t every) {
+    if (!c) return fail(TSMI_E_INVALID, "tsmi_evolve_checkpointed: null context");
+    c->ck_valid = false;
+    if (n < 0 || every < 1) return fail(TSMI_E_INVALID, "tsmi_evolve_checkpointed: need n >= 0 and every >= 1");
+    if (!c->has_state || !c->has_e)
+        return fail(TSMI_E_INVALID, "tsmi_evolve_checkpointed: state and exp potential must be set");
+    if (!c->adj_ok)
+        return fail(TSMI_E_UNSUPPORTED,
+                    "tsmi_evolve_checkpointed: the reverse sweep needs real time, closed axes and zero couplings");
+    const size_t pb = plane_bytes(c);
+    const int count = n > 0 ? (n - 1) / every + 1 : 0;
+    const size_t need = 2 * (size_t)count * pb;
+    if (need > c->ck_cap) {
+        if (c->ck) TSMI_HIP(hipFree(c->ck));
+        c->ck = nullptr;
+        c->ck_cap = 0;
+        TSMI_HIP(hipMalloc(&c->ck, need));
+        c->ck_cap = need;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (i % every == 0) {
+            char *dst = (char *)c->ck + (size_t)(i / every) * 2 * pb;
+            TSMI_HIP(hipMemcpyAsync(dst, c->psi_re[c->cur], pb, hipMemcpyDeviceToDevice, c->stream));
+            TSMI_HIP(hipMemcpyAsync(dst + pb, c->psi_im[c->cur], pb, hipMemcpyDeviceToDevice, c->stream));
+        }
+        const int st = step_once(c, 0);
+        if (st) return st;
+    }
+    c->ck_every = every;
+    c->ck_steps = n;
+    c->ck_count = count;
+    c->ck_valid = true;
+    return TSMI_OK;
+}
+
+int tsmi_adjoint_device(tsmi_ctx *c, const double *g_re, const double *g_im, double *grad_v, double *g0_re,
+                        double *g0_im) {
+    if (!c || !g_re || !g_im || !grad_v) return fail(TSMI_E_INVALID, "tsmi_adjoint_device: null argument");
+    if (!c->ck_valid) return fail(TSMI_E_INVALID, "tsmi_adjoint_device: needs a preceding tsmi_evolve_checkpointed");
+    void *l_re = nullptr, *l_im = nullptr;
+    int st = reverse_sweep(c, g_re, g_im, &l_re, &l_im);
+    if (st) return st;
+    TSMI_HIP(hipMemcpy2DAsync(grad_v, (size_t)c->W * 8, c->grad, (size_t)c->pitch * 8, (size_t)c->W * 8, c->H,
+                              hipMemcpyDeviceToDevice, c->stream));
+    if (g0_re) st = export_device(c, l_re, g0_re);
+    if (!st && g0_im) st = export_device(c, l_im, g0_im);
+    return st;
+}
+
+int tsmi_adjoint(tsmi_ctx *c, const double *g_re, const double *g_im, double *grad_v, double *g0_re, double *g0_im) {
+    if (!c || !g_re || !g_im || !grad_v) return fail(TSMI_E_INVALID, "tsmi_adjoint: null argument");
+    if (!c->ck_valid) return fail(TSMI_E_INVALID, "tsmi_adjoint: needs a preceding tsmi_evolve_checkpointed");
+    const size_t n = (size_t)c->W * (size_t)c->H;
+    double *tmp = nullptr;
+    TSMI_HIP(hipMalloc(&tmp, 5 * n * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(tmp, g_re, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp + n, g_im, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    int st = e == hipSuccess ? TSMI_OK : fail(TSMI_E_HIP, std::string("tsmi_adjoint upload: ") + hipGetErrorString(e));
+    if (!st) st = tsmi_adjoint_device(c, tmp, tmp + n, tmp + 2 * n, tmp + 3 * n, tmp + 4 * n);
+    if (!st) {
+        e = hipMemcpyAsync(grad_v, tmp + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && g0_re) e = hipMemcpyAsync(g0_re, tmp + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && g0_im) e = hipMemcpyAsync(g0_im, tmp + 4 * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) st = fail(TSMI_E_HIP, std::string("tsmi_adjoint download: ") + hipGetErrorString(e));
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    return st;
 }
 
 int tsmi_squared_norm(tsmi_ctx *c, double *out) {

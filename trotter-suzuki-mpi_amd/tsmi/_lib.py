@@ -19,7 +19,10 @@ TSMI_OK = 0
 EXPORTS = (
     "tsmi_create", "tsmi_destroy", "tsmi_set_stream", "tsmi_set_state", "tsmi_set_state_device",
     "tsmi_set_exp_potential", "tsmi_set_exp_potential_device", "tsmi_set_potential",
-    "tsmi_set_potential_device", "tsmi_evolve", "tsmi_step", "tsmi_squared_norm", "tsmi_energy", "tsmi_expected_values",
+    "tsmi_set_potential_device", "tsmi_evolve", "tsmi_step", "tsmi_evolve_checkpointed", "tsmi_adjoint",
+    "tsmi_adjoint_device", "tsmi_batch_create", "tsmi_batch_destroy", "tsmi_batch_set_state",
+    "tsmi_batch_set_exp_potential", "tsmi_batch_evolve", "tsmi_batch_get_state", "tsmi_batch_tune",
+    "tsmi_batch_info", "tsmi_batch_synchronize", "tsmi_squared_norm", "tsmi_energy", "tsmi_expected_values",
     "tsmi_get_sample", "tsmi_get_state_device", "tsmi_synchronize", "tsmi_geometry", "tsmi_last_error",
 )
 
@@ -66,6 +69,19 @@ def load():
     L.tsmi_get_state_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.tsmi_evolve.argtypes = [C.c_void_p, C.c_int32]
     L.tsmi_step.argtypes = [C.c_void_p, C.c_int32]
+    L.tsmi_evolve_checkpointed.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    L.tsmi_adjoint.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+    L.tsmi_adjoint_device.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+    L.tsmi_batch_create.argtypes = [C.POINTER(Desc), C.c_int32, C.POINTER(C.c_void_p)]
+    L.tsmi_batch_destroy.argtypes = [C.c_void_p]
+    L.tsmi_batch_destroy.restype = None
+    L.tsmi_batch_set_state.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
+    L.tsmi_batch_set_exp_potential.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
+    L.tsmi_batch_evolve.argtypes = [C.c_void_p, C.c_int32]
+    L.tsmi_batch_get_state.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
+    L.tsmi_batch_tune.argtypes = [C.c_void_p, C.c_int32, _dp]
+    L.tsmi_batch_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
+    L.tsmi_batch_synchronize.argtypes = [C.c_void_p]
     L.tsmi_squared_norm.argtypes = [C.c_void_p, _dp]
     L.tsmi_energy.argtypes = [C.c_void_p, _dp]
     L.tsmi_expected_values.argtypes = [C.c_void_p, _dp]
@@ -75,7 +91,7 @@ def load():
     L.tsmi_last_error.argtypes = []
     L.tsmi_last_error.restype = C.c_char_p
     for name in EXPORTS:
-        if name not in ("tsmi_destroy", "tsmi_last_error"):
+        if name not in ("tsmi_destroy", "tsmi_batch_destroy", "tsmi_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -32,6 +32,7 @@ class HipKernel:
                  norm_target: float = 0.0, tile_rows: int = 0):
         L = _lib.load()
         self.lattice = lattice
+        self.delta_t = float(delta_t)
         self.imag_time = bool(imag_time)
         self.precision = int(precision)
         d = _lib.Desc(lattice.dim_x, lattice.dim_y, int(lattice.periodic_x), int(lattice.periodic_y),
@@ -103,6 +104,29 @@ class HipKernel:
 
     def evolve(self, n_steps: int) -> None:
         _lib.check(self._L.tsmi_evolve(self._h, int(n_steps)), "tsmi_evolve")
+
+    # -- gradient with respect to the potential (real time, closed, g = 0) -------
+    def evolve_checkpointed(self, n_steps: int, every: int = 100) -> None:
+        """evolve(n_steps), keeping the state every ``every`` steps for adjoint()."""
+        _lib.check(self._L.tsmi_evolve_checkpointed(self._h, int(n_steps), int(every)), "tsmi_evolve_checkpointed")
+
+    def adjoint(self, g_final: np.ndarray):
+        """Reverse sweep from G = dL/dRe psi_N + i dL/dIm psi_N; returns
+        (dL/dV, dL/dRe psi_0 + i dL/dIm psi_0) on the tile."""
+        gr, gi = self._plane(g_final.real), self._plane(g_final.imag)
+        gv = np.zeros_like(gr)
+        g0r = np.zeros_like(gr)
+        g0i = np.zeros_like(gr)
+        _lib.check(self._L.tsmi_adjoint(self._h, _lib.dptr(gr), _lib.dptr(gi), _lib.dptr(gv), _lib.dptr(g0r),
+                                        _lib.dptr(g0i)), "tsmi_adjoint")
+        return gv, g0r + 1j * g0i
+
+    def adjoint_device(self, g_re_ptr: int, g_im_ptr: int, grad_ptr: int, g0_re_ptr: int | None = None,
+                       g0_im_ptr: int | None = None) -> None:
+        """adjoint() on device W*H FP64 arrays; asynchronous on the context stream."""
+        _lib.check(self._L.tsmi_adjoint_device(self._h, C.c_void_p(g_re_ptr), C.c_void_p(g_im_ptr),
+                                               C.c_void_p(grad_ptr), C.c_void_p(g0_re_ptr or 0),
+                                               C.c_void_p(g0_im_ptr or 0)), "tsmi_adjoint_device")
 
     def synchronize(self) -> None:
         _lib.check(self._L.tsmi_synchronize(self._h), "tsmi_synchronize")
