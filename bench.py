@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--precision", choices=["both", "32", "64"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=12, help="timed steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-studies", action="store_true",
+                    help="skip the precision study, the batch and the adjoint measurements")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (for counter calibration) and exit")
     return ap.parse_args()
@@ -116,6 +118,106 @@ def cpu_baseline(lat, psi, v, params, steps):
     s = time.perf_counter() - t
     return {"value": lat.dim_x * lat.dim_y * steps / s, "unit": "site-updates/s", "cores": threads, "kind": "port",
             "sample": "oracle restatement (OpenMP over rows), " + sample}
+
+
+def precision_study(steps: int = 10000):
+    """Imaginary-time GPE ground state, 2048^2 closed, g=100, dt=1e-4, FP32 bulk
+    against FP64 on the GPU (the FP64 path agrees with the oracle to 1e-12)."""
+    import numpy as np
+    import torch
+    from tsmi import workloads as W
+    from tsmi.hip import kernel_for
+    wl = W.config1(steps=steps)
+    lat = wl.lattice
+    nt = W.squared_norm(lat, wl.psi)
+    out = {}
+    for prec in (64, 32):
+        k = kernel_for(wl, precision=prec, norm_target=nt)
+        try:
+            k.synchronize()
+            t = time.perf_counter()
+            k.evolve(steps)
+            k.synchronize()
+            sec = time.perf_counter() - t
+            out[prec] = dict(psi=k.sample(), energy=k.energy(), sec=sec)
+        finally:
+            k.close()
+    e64, e32 = out[64]["energy"], out[32]["energy"]
+    iy, ix = lat.inner_y, lat.inner_x
+    a32, a64 = out[32]["psi"][iy, ix], out[64]["psi"][iy, ix]
+    rel = float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64))
+    sites = lat.dim_x * lat.dim_y
+    return {"workload": f"imaginary-time GPE ground state, {lat.dim_x}x{lat.dim_y} closed, side 24, g=100, "
+                        f"dt=1e-4, {steps} steps",
+            "fp64": {"energy": e64[0], "norm": e64[5], "seconds": out[64]["sec"],
+                     "site_updates_per_s": sites * steps / out[64]["sec"]},
+            "fp32": {"energy": e32[0], "norm": e32[5], "seconds": out[32]["sec"],
+                     "site_updates_per_s": sites * steps / out[32]["sec"],
+                     "energy_rel_err_vs_fp64": abs(e32[0] - e64[0]) / abs(e64[0]), "rel_l2_vs_fp64": rel}}
+
+
+def batch_study(steps: int = 500, count: int = 512):
+    """512 independent lattices of seeded sides in {64..1024}, mixed closed and
+    periodic, advanced together in FP32 (one persistent launch per step,
+    graph-captured), after the autotuner."""
+    from tsmi import workloads as W
+    from tsmi.batch import HipBatch
+    t = time.perf_counter()
+    wls = W.config3(count=count, steps=steps)
+    build = time.perf_counter() - t
+    sites = sum(w.lattice.dim_x * w.lattice.dim_y for w in wls)
+    with HipBatch(wls, precision=32) as b:
+        b.synchronize()
+        ms_tuned = b.tune(steps=3)
+        b.evolve(steps)          # records the graph
+        b.synchronize()
+        t = time.perf_counter()
+        b.evolve(steps)
+        b.synchronize()
+        sec = time.perf_counter() - t
+        rows = sorted({(b.info(i)["H"], b.info(i)["rows_per_segment"]) for i in range(len(wls))})
+    return {"workload": f"{count} real-time lattices, sides 64..1024 (seeded), mixed closed/periodic, FP32, "
+                        f"{steps} graph-captured steps", "inner_sites": sites,
+            "site_updates_per_s": sites * steps / sec, "ms_per_step": sec * 1e3 / steps,
+            "tuned_ms_per_step": ms_tuned, "rows_per_segment_by_tile_height": rows, "host_build_s": build}
+
+
+def adjoint_study():
+    """dL/dV of L = 1 - |<target|psi(T)>|^2 on the 1024^2 closed lattice, 2000
+    real-time steps, checkpoints every 100, FP32 and FP64, checked against the
+    reference solver's central differences (tests/golden/adjoint_config4.npz)."""
+    import numpy as np
+    from tsmi import workloads as W
+    from tsmi.hip import kernel_for
+    from tsmi.losses import infidelity
+    wl, target = W.config4()
+    fx = os.path.join(ROOT, "tests", "golden", "adjoint_config4.npz")
+    fd = np.load(fx) if os.path.exists(fx) else None
+    out = {"workload": "1024x1024 closed, 16 Gaussians, 2000 steps dt=1e-3, checkpoints every 100"}
+    grads = {}
+    for prec in (64, 32):
+        k = kernel_for(wl, precision=prec)
+        try:
+            k.synchronize()
+            t = time.perf_counter()
+            k.evolve_checkpointed(wl.steps, 100)
+            k.synchronize()
+            fwd = time.perf_counter() - t
+            L, g = infidelity(wl.lattice, target, k.sample())
+            t = time.perf_counter()
+            gv, _ = k.adjoint(g)
+            rev = time.perf_counter() - t
+        finally:
+            k.close()
+        grads[prec] = gv
+        o = {"loss": L, "forward_s": fwd, "reverse_s": rev}
+        if fd is not None:
+            pts = fd["points"]
+            ref = fd["fd"]
+            o["max_rel_err_vs_reference_fd"] = float(np.max(np.abs(gv.ravel()[pts] - ref)) / np.max(np.abs(ref)))
+        out[f"fp{prec}"] = o
+    out["fp32_rel_err_vs_fp64"] = float(np.max(np.abs(grads[32] - grads[64])) / np.max(np.abs(grads[64])))
+    return out
 
 
 def main():
@@ -209,6 +311,12 @@ def main():
         line["fp32_over_fp64"] = r["su"] / r64["su"]
         line["fp32_rel_l2_vs_fp64"] = rel
         line["norm_drift"] = {"fp32": abs(results[32]["norm"] - 1.0), "fp64": abs(r64["norm"] - 1.0)}
+    if not args.no_studies:
+        for name, fn in (("precision_study", precision_study), ("batch", batch_study), ("adjoint", adjoint_study)):
+            log(f"{name} ...")
+            t = time.perf_counter()
+            line[name] = fn()
+            log(f"{name}: done in {time.perf_counter() - t:.1f} s")
     print(json.dumps(line), flush=True)
 
 

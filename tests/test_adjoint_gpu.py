@@ -124,3 +124,30 @@ def test_torch_autograd_matches_oracle(oracle):
         assert rel_max(p_re.grad.cpu().numpy() + 1j * p_im.grad.cpu().numpy(), ref_0) < 1e-10
     finally:
         k.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_config4_gradient_against_reference_fd(precision):
+    """The 1024^2 differentiation workload (2000 steps, checkpoints every 100)
+    against central differences of the reference solver at 32 cells
+    (tests/golden/adjoint_config4.npz, written by make_adjoint_golden.py).
+    Bar: max |dL/dV - FD| over the cells <= 1e-4 of the largest |FD|; the
+    oracle's FP64 adjoint meets it at 2.9e-8 (largest gap from the FD noise of
+    the smallest entries)."""
+    import os
+    from golden_util import input_digest
+    from tsmi.hip import kernel_for
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "adjoint_config4.npz"))
+    wl, target = W.config4()
+    assert str(z["input_sha256"]) == input_digest(wl)
+    k = kernel_for(wl, precision=precision)
+    try:
+        k.evolve_checkpointed(wl.steps, 100)
+        L, g = infidelity(wl.lattice, target, k.sample())
+        gv, _ = k.adjoint(g)
+    finally:
+        k.close()
+    assert abs(L - float(z["loss0"])) < (1e-12 if precision == 64 else 1e-5)
+    fd = z["fd"]
+    err = float(np.max(np.abs(gv.ravel()[z["points"]] - fd)) / np.max(np.abs(fd)))
+    assert err < 1e-4, err

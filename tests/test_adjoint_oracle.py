@@ -74,3 +74,15 @@ def test_oracle_adjoint_rejects_unsupported(oracle):
     wl = W.small_case(16, 16, False, False, steps=3, g=1.0)
     with pytest.raises(ValueError):
         oracle.adjoint(wl, np.zeros_like(wl.psi))
+
+
+def test_config4_fixture_matches_builder():
+    """The reference finite differences were taken on exactly the inputs the
+    builder makes today."""
+    import os
+    from golden_util import input_digest
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "adjoint_config4.npz"))
+    wl, _ = W.config4()
+    assert str(z["input_sha256"]) == input_digest(wl)
+    assert len(set(z["points"].tolist())) == 32 and float(z["h"]) == 1e-4
+    assert np.all(np.isfinite(z["fd"])) and np.max(np.abs(z["fd"])) > 1e-6

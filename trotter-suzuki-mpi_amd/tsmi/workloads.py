@@ -142,6 +142,37 @@ def config2(n: int = 8192, steps: int = 1000, length: float = 64.0, n_vortices: 
     return Workload("config2", lat, psi, v, 5e-4, steps, g=g, extra=dict(mu=mu, r_tf=r_tf))
 
 
+CONFIG3_SIDES = (64, 128, 256, 512, 1024)
+
+
+def config3_specs(count: int = 512, sides=CONFIG3_SIDES):
+    """The seeded shape of the batching workload: per lattice (side, periodic),
+    side uniform over ``sides``, both axes periodic with probability 1/2."""
+    u = uniform(SEED_BASE + 3, 2 * count)
+    out = []
+    for i in range(count):
+        side = sides[min(int(u[2 * i] * len(sides)), len(sides) - 1)]
+        out.append((side, bool(u[2 * i + 1] < 0.5)))
+    return out
+
+
+def config3_state(i: int, side: int, periodic: bool, steps: int = 500, length: float = 20.0) -> Workload:
+    """Lattice i of the batching workload: V a seeded sum of 8 Gaussians, a
+    seeded Gaussian packet as in config0, dt=1e-3, real time."""
+    seed = SEED_BASE + 3000 + i
+    lat = Lattice2D(side, length, periodic_x=periodic, periodic_y=periodic)
+    x0, y0, kx, ky = uniform(seed, 4, lo=-2.0, hi=2.0)
+    psi = normalise(lat, gaussian_packet(lat, x0, y0, kx, ky))
+    v = gaussian_sum(lat, seed, 8, offset=4)
+    return Workload(f"config3_{i}", lat, psi, v, 1e-3, steps)
+
+
+def config3(count: int = 512, steps: int = 500, sides=CONFIG3_SIDES, length: float = 20.0):
+    """Batching: ``count`` independent real-time Schroedinger lattices of seeded
+    sides and boundaries (list of Workloads)."""
+    return [config3_state(i, s, p, steps, length) for i, (s, p) in enumerate(config3_specs(count, sides))]
+
+
 def config4(n: int = 1024, steps: int = 2000, length: float = 20.0):
     """Differentiation: closed linear Schroedinger with V a seeded sum of 16
     Gaussians, seeded Gaussian packet, target a displaced Gaussian at a seeded
