@@ -23,12 +23,12 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def run_hip(wl, oracle, precision=64, steps=None, tile_rows=0, calls=None):
+def run_hip(wl, oracle, precision=64, steps=None, tile_rows=0, calls=None, sweeps=0):
     from tsmi.hip import kernel_for
     d = oracle.make_desc(wl)
     e = oracle.exp_potential(wl.v, wl.dt, wl.imag_time)
     k = kernel_for(wl, precision=precision, norm_target=d.norm_target if wl.imag_time else 0.0,
-                   tile_rows=tile_rows, e_planes=e)
+                   tile_rows=tile_rows, e_planes=e, sweeps=sweeps)
     try:
         for n in (calls or [wl.steps if steps is None else steps]):
             k.evolve(n)
@@ -151,3 +151,51 @@ def test_fp32_high_beta_accuracy_1000_steps(oracle):
     a, b = outs[32][0][lat.inner_y, lat.inner_x], outs[64][0][lat.inner_y, lat.inner_x]
     assert rel_l2(a, b) < 1e-4
     assert abs(outs[32][1] - outs[64][1]) < 2e-5
+
+
+FUSED = [
+    dict(n_x=64, n_y=64, periodic_x=True, periodic_y=True, g=2.0),
+    dict(n_x=500, n_y=90, periodic_x=True, periodic_y=True, g=1.0, g_lhy=0.5),   # strips across the seam
+    dict(n_x=601, n_y=75, periodic_x=False, periodic_y=False, g=1.0),            # ragged, closed
+    dict(n_x=8, n_y=8, periodic_x=True, periodic_y=True),
+    dict(n_x=13, n_y=7, periodic_x=False, periodic_y=False, g=0.5),
+]
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 4, 5, 8, 13])
+@pytest.mark.parametrize("case", FUSED, ids=lambda c: case_id(c))
+def test_two_steps_per_pass_match_oracle(case, steps, oracle):
+    """Fused passes (two steps per HBM pass) against the oracle, and single
+    steps against the same oracle, for every step-count parity."""
+    wl = W.small_case(steps=steps, **case)
+    ref = oracle.evolve(wl)
+    for sweeps in (2, 1):
+        psi, *_ = run_hip(wl, oracle, sweeps=sweeps)
+        assert rel_l2(psi, ref) < 1e-12, sweeps
+
+
+def test_two_steps_per_pass_split_calls(oracle):
+    """evolve(6) then evolve(7) with fused passes: each call starts from the halo
+    the previous one left, as the reference does."""
+    wl = W.small_case(96, 80, True, True, steps=13, g=2.0)
+    psi, *_ = run_hip(wl, oracle, calls=[6, 7], sweeps=2, tile_rows=16)
+    ref = oracle.evolve(wl, steps=6)
+    ref = oracle.evolve(wl, steps=7, psi=ref)
+    assert rel_l2(psi, ref) < 1e-12
+
+
+def test_two_steps_per_pass_fp32(oracle):
+    wl = W.small_case(256, 256, True, True, steps=40, g=3.0)
+    ref = oracle.evolve(wl)
+    a, *_ = run_hip(wl, oracle, precision=32, sweeps=2)
+    b, *_ = run_hip(wl, oracle, precision=32, sweeps=1)
+    assert rel_l2(a, ref) < 2e-5 and rel_l2(b, ref) < 2e-5
+    assert rel_l2(a, b) < 1e-5
+
+
+def test_two_steps_per_pass_rejected_where_not_exact():
+    from tsmi import _lib
+    from tsmi.hip import kernel_for
+    for wl in (W.small_case(32, 32, True, False, steps=2), W.small_case(32, 32, True, True, steps=2, imag_time=True)):
+        with pytest.raises(_lib.TsmiError, match="two steps per pass"):
+            kernel_for(wl, sweeps=2)

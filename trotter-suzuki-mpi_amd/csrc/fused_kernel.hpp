@@ -1,0 +1,201 @@
+// fused_kernel.hpp -- two real-time steps per pass over HBM (temporal blocking).
+//
+// The step kernel's strip/segment decomposition with an 8-cell apron on each
+// side (the dependency cone of two steps) and two register pipelines in
+// series: pipeline A runs the nine sub-passes of step n on the input rows, and
+// every row it emits (4 rows behind its input) is the input row of pipeline B,
+// which runs step n+1 and emits 8 rows behind the input.  Each state row is read
+// once and written once for two steps; the E row is read by both pipelines, 4
+// rows apart (the second read is served by the caches).
+//
+// Two steps on a tile with closed edges equal two single steps exactly when no
+// halo exchange is due between them: on closed axes always; on periodic axes
+// the host keeps an 8-wide halo, refreshes it before each pass (so the 4-wide
+// reference halo the second step reads holds the wrapped values the exchange
+// would have written) and gives the pass the fully wrapped E table.  Real time
+// only: imaginary time renormalises between steps.
+#pragma once
+#include "step_kernel.hpp"
+
+namespace tsmi {
+
+template <typename T, int VEC>
+struct Pipe {
+    Row<T, VEC> p1, p2, p3, p4;
+};
+
+// One beat of a real-time step pipeline (the generic form of step_wave's beat):
+// input row t (in), E row t-2 (e), row validity v[k] = valid(t-k); returns row
+// t-4 in `out`.
+template <int TP, bool LHY, typename T, int VEC>
+__device__ __forceinline__ void pipe_beat(Pipe<T, VEC> &q, Row<T, VEC> in, const Row<T, VEC> &e, Row<T, VEC> &out,
+                                          bool v0, bool v1, bool v2, bool v3, bool v4, const HCoef<T, VEC> &hc,
+                                          T mV, T tV, T unV, T c, T cl) {
+    const T a1 = (v0 && v1) ? mV : unV, b1 = (v0 && v1) ? tV : T(0);
+    const T a2 = (v1 && v2) ? mV : unV, b2 = (v1 && v2) ? tV : T(0);
+    const T a3 = (v2 && v3) ? mV : unV, b3 = (v2 && v3) ? tV : T(0);
+    const T a4 = (v3 && v4) ? mV : unV, b4 = (v3 && v4) ? tV : T(0);
+    vpair<(TP + 1) & 1, false>(q.p1, in, a1, b1);
+    Row<T, VEC> o1 = q.p1;
+    q.p1 = in;
+    hpass<(TP + 1) & 1, false>(o1, hc);
+    vpair<(TP + 1) & 1, false>(q.p2, o1, a2, b2);
+    Row<T, VEC> o2 = q.p2;
+    q.p2 = o1;
+    hpass<(TP + 1) & 1, false>(o2, hc);
+    ppass<false, LHY>(o2, e, c, cl);
+    hpass<(TP + 1) & 1, false>(o2, hc);
+    vpair<TP & 1, false>(q.p3, o2, a3, b3);
+    Row<T, VEC> o3 = q.p3;
+    q.p3 = o2;
+    hpass<(TP + 1) & 1, false>(o3, hc);
+    vpair<TP & 1, false>(q.p4, o3, a4, b4);
+    out = q.p4;
+    q.p4 = o3;
+}
+
+// The same beat when the strip is interior and all five rows are valid.
+template <int TP, bool LHY, int MODE, typename T, int VEC>
+__device__ __forceinline__ void pipe_beat_fast(Pipe<T, VEC> &q, Row<T, VEC> in, const Row<T, VEC> &e,
+                                               Row<T, VEC> &out, T kH, T kV, T c, T cl) {
+    constexpr int M = MODE < 0 ? 0 : MODE;
+    vpair_fast<(TP + 1) & 1, false, M>(q.p1, in, kV);
+    Row<T, VEC> o1 = q.p1;
+    q.p1 = in;
+    hpass_fast<(TP + 1) & 1, false, M>(o1, kH);
+    vpair_fast<(TP + 1) & 1, false, M>(q.p2, o1, kV);
+    Row<T, VEC> o2 = q.p2;
+    q.p2 = o1;
+    hpass_fast<(TP + 1) & 1, false, M>(o2, kH);
+    ppass<false, LHY>(o2, e, c, cl);
+    hpass_fast<(TP + 1) & 1, false, M>(o2, kH);
+    vpair_fast<TP & 1, false, M>(q.p3, o2, kV);
+    Row<T, VEC> o3 = q.p3;
+    q.p3 = o2;
+    hpass_fast<(TP + 1) & 1, false, M>(o3, kH);
+    vpair_fast<TP & 1, false, M>(q.p4, o3, kV);
+    out = q.p4;
+    q.p4 = o3;
+}
+
+// Two real-time steps of one wavefront's strip segment.  p.seg_rows, p.n_strips
+// and p.n_waves describe the fused decomposition (strips of kWave*VEC - 16
+// output columns).
+template <typename T, int VEC, bool LHY, int MODE>
+__device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wave, const int lane) {
+    constexpr int AP = 2 * kApron;
+    constexpr int SW = kWave * VEC - 2 * AP;
+    const int strip = wave % p.n_strips;
+    const int seg = wave / p.n_strips;
+    const int X0 = strip * SW - AP;
+    const int xl = X0 + lane * VEC;
+    const int y_out_lo = seg * p.seg_rows;
+    const int y_out_hi = min(p.H, y_out_lo + p.seg_rows);
+    const int y_lo = max(0, y_out_lo - AP);
+    const int y_hi = min(p.H, y_out_hi + AP);
+    const bool ld_ok = xl >= 0 && xl + VEC <= p.pitch;
+    const bool own = lane >= AP / VEC && lane < kWave - AP / VEC;
+
+    const HCoef<T, VEC> hc = make_hcoef<T, VEC>(xl, lane, p.W, p.mH, p.tH, p.unH);
+    const T mV = p.mV, tV = p.tV, unV = p.unV;
+    const T kH = MODE == 0 ? p.tH : p.mH, kV = MODE == 0 ? p.tV : p.mV;
+    const T c = p.c_nl, cl = p.c_lhy;
+    const bool interior = MODE >= 0 && X0 >= 0 && X0 + kWave * VEC <= p.W;
+
+    const long seg0 = (long)y_lo * p.pitch;
+    const unsigned seg_bytes = (unsigned)((long)(y_hi - y_lo) * p.pitch * (long)sizeof(T));
+    const __amdgpu_buffer_rsrc_t r_in_re = make_rsrc(p.in_re + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_in_im = make_rsrc(p.in_im + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_e_re = make_rsrc(p.e_re + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_e_im = make_rsrc(p.e_im + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_out_re = make_rsrc(p.out_re + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_out_im = make_rsrc(p.out_im + seg0, seg_bytes);
+    const unsigned row_bytes = (unsigned)p.pitch * (unsigned)sizeof(T);
+    const unsigned v_ld = ld_ok ? (unsigned)xl * (unsigned)sizeof(T) : kOffNone;
+    const unsigned v_st = (own && ld_ok) ? (unsigned)xl * (unsigned)sizeof(T) : kOffNone;
+
+    auto load_row = [&](__amdgpu_buffer_rsrc_t rr, __amdgpu_buffer_rsrc_t ri, int y, Row<T, VEC> &r) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_load<T, VEC>(rr, v_ld, so, r.re);
+        buf_load<T, VEC>(ri, v_ld, so, r.im);
+    };
+    auto valid = [&](int y) { return y >= y_lo && y < y_hi; };
+
+    Pipe<T, VEC> A, B;
+    Row<T, VEC> inA, inB, eA, eB, fA, fB;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        A.p1.re[k] = A.p1.im[k] = A.p2.re[k] = A.p2.im[k] = T(0);
+        A.p3.re[k] = A.p3.im[k] = A.p4.re[k] = A.p4.im[k] = T(0);
+        B.p1.re[k] = B.p1.im[k] = B.p2.re[k] = B.p2.im[k] = T(0);
+        B.p3.re[k] = B.p3.im[k] = B.p4.re[k] = B.p4.im[k] = T(0);
+        inA.re[k] = inA.im[k] = inB.re[k] = inB.im[k] = T(0);
+        eA.re[k] = eA.im[k] = eB.re[k] = eB.im[k] = T(0);
+        fA.re[k] = fA.im[k] = fB.re[k] = fB.im[k] = T(0);
+    }
+
+    auto emit = [&](int y, const Row<T, VEC> &o) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_store<T, VEC>(r_out_re, v_st, so, o.re);
+        buf_store<T, VEC>(r_out_im, v_st, so, o.im);
+    };
+
+    // beat t: pipeline A takes input row t (E row t-2) and gives row t-4, the
+    // input of pipeline B (E row t-6), which gives row t-8
+    auto beat = [&](auto tp, int t, const Row<T, VEC> &in, const Row<T, VEC> &ea, const Row<T, VEC> &eb) {
+        constexpr int TP = decltype(tp)::value;
+        Row<T, VEC> a, b;
+        pipe_beat<TP, LHY>(A, in, ea, a, valid(t), valid(t - 1), valid(t - 2), valid(t - 3), valid(t - 4), hc, mV, tV,
+                           unV, c, cl);
+        pipe_beat<TP, LHY>(B, a, eb, b, valid(t - 4), valid(t - 5), valid(t - 6), valid(t - 7), valid(t - 8), hc, mV,
+                           tV, unV, c, cl);
+        const int y = t - 8;
+        if (valid(y) && y >= y_out_lo && y < y_out_hi) emit(y, b);
+    };
+    auto beat_fast = [&](auto tp, int t, const Row<T, VEC> &in, const Row<T, VEC> &ea, const Row<T, VEC> &eb) {
+        constexpr int TP = decltype(tp)::value;
+        Row<T, VEC> a, b;
+        pipe_beat_fast<TP, LHY, MODE>(A, in, ea, a, kH, kV, c, cl);
+        pipe_beat_fast<TP, LHY, MODE>(B, a, eb, b, kH, kV, c, cl);
+        const int y = t - 8;
+        if (y >= y_out_lo && y < y_out_hi) emit(y, b);
+    };
+
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    int t = y_lo;
+    if (t < y_hi) load_row(r_in_re, r_in_im, t, inA);
+    if (t + 1 < y_hi) load_row(r_in_re, r_in_im, t + 1, inB);
+    const int t_end = y_hi + 8;
+    for (; t < t_end; t += 2) {
+        if (interior && t >= y_lo + 8 && t + 1 < y_hi) {
+            beat_fast(I0(), t, inA, eA, fA);
+            if (t + 2 < y_hi) load_row(r_in_re, r_in_im, t + 2, inA);
+            load_row(r_e_re, r_e_im, t, eA);
+            load_row(r_e_re, r_e_im, t - 4, fA);
+            beat_fast(I1(), t + 1, inB, eB, fB);
+            if (t + 3 < y_hi) load_row(r_in_re, r_in_im, t + 3, inB);
+            load_row(r_e_re, r_e_im, t + 1, eB);
+            load_row(r_e_re, r_e_im, t - 3, fB);
+        } else {
+            beat(I0(), t, inA, eA, fA);
+            if (t + 2 < y_hi) load_row(r_in_re, r_in_im, t + 2, inA);
+            if (t < y_hi) load_row(r_e_re, r_e_im, t, eA);
+            if (valid(t - 4)) load_row(r_e_re, r_e_im, t - 4, fA);
+            beat(I1(), t + 1, inB, eB, fB);
+            if (t + 3 < y_hi) load_row(r_in_re, r_in_im, t + 3, inB);
+            if (t + 1 < y_hi) load_row(r_e_re, r_e_im, t + 1, eB);
+            if (valid(t - 3)) load_row(r_e_re, r_e_im, t - 3, fB);
+        }
+    }
+}
+
+template <typename T, int VEC, bool LHY, int MODE>
+__global__ void __launch_bounds__(256) step2_kernel(StepParams<T> p) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
+    if (wave >= p.n_waves) return;
+    step2_wave<T, VEC, LHY, MODE>(p, wave, lane);
+}
+
+}  // namespace tsmi

@@ -411,36 +411,54 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, un
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
 }
 
+typedef unsigned __attribute__((ext_vector_type(2))) u32x2;
+
+// A lane's VEC consecutive values: 16-byte pieces, or one 8-byte piece.
 template <typename T, int VEC>
 __device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, T (&v)[VEC]) {
-    static_assert(sizeof(T) * VEC % 16 == 0, "rows are loaded in 16-byte pieces");
-    constexpr int P = sizeof(T) * VEC / 16;
-    constexpr int E = 16 / sizeof(T);
+    static_assert(sizeof(T) * VEC % 16 == 0 || sizeof(T) * VEC == 8, "rows are loaded in 16- or 8-byte pieces");
+    if constexpr (sizeof(T) * VEC == 8) {
+        typedef T __attribute__((ext_vector_type(VEC))) TV;
+        const TV x = __builtin_bit_cast(TV, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 #pragma unroll
-    for (int i = 0; i < P; ++i) {
-        const u32x4 w = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * i, soff, 0));
-        typedef T __attribute__((ext_vector_type(E))) TV;
-        const TV x = __builtin_bit_cast(TV, w);
+        for (int k = 0; k < VEC; ++k) v[k] = x[k];
+    } else {
+        constexpr int P = sizeof(T) * VEC / 16;
+        constexpr int E = 16 / sizeof(T);
 #pragma unroll
-        for (int k = 0; k < E; ++k) v[i * E + k] = x[k];
+        for (int i = 0; i < P; ++i) {
+            const u32x4 w = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * i, soff, 0));
+            typedef T __attribute__((ext_vector_type(E))) TV;
+            const TV x = __builtin_bit_cast(TV, w);
+#pragma unroll
+            for (int k = 0; k < E; ++k) v[i * E + k] = x[k];
+        }
     }
 }
 
+// The row offset goes into the vector offset with a zero scalar offset: a
+// 128-bit store then carries a one-wait-state hazard on its data registers
+// that the compiler models (it does not for a store with a scalar-register
+// offset).
 template <typename T, int VEC>
 __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const T (&v)[VEC]) {
-    constexpr int P = sizeof(T) * VEC / 16;
-    constexpr int E = 16 / sizeof(T);
-#pragma unroll
-    for (int i = 0; i < P; ++i) {
-        typedef T __attribute__((ext_vector_type(E))) TV;
+    if constexpr (sizeof(T) * VEC == 8) {
+        typedef T __attribute__((ext_vector_type(VEC))) TV;
         TV x;
 #pragma unroll
-        for (int k = 0; k < E; ++k) x[k] = v[i * E + k];
-        // the row offset goes into the vector offset with a zero scalar offset:
-        // a 128-bit store then carries a one-wait-state hazard on its data
-        // registers that the compiler models (it does not for a store with a
-        // scalar-register offset)
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, voff + soff + 16 * i, 0, 0);
+        for (int k = 0; k < VEC; ++k) x[k] = v[k];
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, voff + soff, 0, 0);
+    } else {
+        constexpr int P = sizeof(T) * VEC / 16;
+        constexpr int E = 16 / sizeof(T);
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            typedef T __attribute__((ext_vector_type(E))) TV;
+            TV x;
+#pragma unroll
+            for (int k = 0; k < E; ++k) x[k] = v[i * E + k];
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, voff + soff + 16 * i, 0, 0);
+        }
     }
 }
 

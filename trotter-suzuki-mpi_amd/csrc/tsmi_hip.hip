@@ -20,6 +20,7 @@
 #include "adjoint_kernel.hpp"
 #include "common.hpp"
 #include "frame.hpp"
+#include "fused_kernel.hpp"
 #include "observables.hpp"
 #include "step_kernel.hpp"
 
@@ -79,6 +80,17 @@ namespace {
     } while (0)
 
 constexpr int kBlock = 256;          // 4 wavefronts per workgroup
+
+// FP32 columns per lane of the single-step and of the fused two-step kernel
+// (FP64 uses 2: one 16-byte piece per plane).  Compile-time tuning constants.
+#ifndef TSMI_FP32_VEC
+#define TSMI_FP32_VEC 4
+#endif
+#ifndef TSMI_FP32_VEC2
+#define TSMI_FP32_VEC2 4
+#endif
+constexpr int kVec32 = TSMI_FP32_VEC;
+constexpr int kVec32F = TSMI_FP32_VEC2;
 constexpr int kObsBlocksMax = 1024;
 
 }  // namespace
@@ -87,8 +99,21 @@ struct tsmi_ctx {
     tsmi_desc d;
     int W = 0, H = 0, pitch = 0, esz = 0, vec = 0;
     int n_strips = 0, n_segs = 0, seg_rows = 0, n_waves = 0;
+    // Views at the origin of the reference tile (W x H, row pitch `pitch`).  With
+    // the 8-wide halo layout they sit 4 rows and 4 columns inside allocations of
+    // Wd x Hd = (W + 8) x (H + 8); otherwise view and allocation coincide.
     void *psi_re[2] = {nullptr, nullptr}, *psi_im[2] = {nullptr, nullptr};
     void *e_re = nullptr, *e_im = nullptr;
+    void *psi_base[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    void *e_base[2] = {nullptr, nullptr};
+    void *e8[2] = {nullptr, nullptr};     // E wrapped over the whole 8-wide halo (fused passes)
+    double *v_base = nullptr;
+    int Wd = 0, Hd = 0;
+    size_t off_el = 0;                    // elements from allocation to view origin
+    bool halo8 = false;
+    // fused two-step passes (fused_kernel.hpp)
+    int sweeps = 1;
+    int n_strips2 = 0, n_segs2 = 0, seg_rows2 = 0, n_waves2 = 0;
     double *v = nullptr, *xc = nullptr, *yc = nullptr;
     double *div = nullptr;            // two divisor slots (imaginary time)
     double *partials = nullptr;       // one per wave
@@ -119,7 +144,7 @@ struct tsmi_ctx {
 
 namespace {
 
-size_t plane_bytes(const tsmi_ctx *c) { return (size_t)c->pitch * (size_t)c->H * (size_t)c->esz; }
+size_t plane_bytes(const tsmi_ctx *c) { return (size_t)c->pitch * (size_t)c->Hd * (size_t)c->esz; }
 
 // map_lattice_to_coordinate_space for one axis (src/common.cpp:45-65, Cartesian)
 void map_axis(int start, double delta, int n, double length, int count, std::vector<double> &out) {
@@ -134,12 +159,11 @@ void map_axis(int start, double delta, int n, double length, int count, std::vec
 }
 
 int free_all(tsmi_ctx *c) {
-    for (int b = 0; b < 2; ++b) {
-        if (c->psi_re[b]) (void)hipFree(c->psi_re[b]);
-        if (c->psi_im[b]) (void)hipFree(c->psi_im[b]);
-    }
-    void *ptrs[] = {c->e_re, c->e_im, c->v, c->xc, c->yc, c->div, c->partials, c->counter, c->obs_partials,
-                    c->ck, c->adj, c->grad};
+    for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < 2; ++k)
+            if (c->psi_base[b][k]) (void)hipFree(c->psi_base[b][k]);
+    void *ptrs[] = {c->e_base[0], c->e_base[1], c->e8[0], c->e8[1], c->v_base, c->xc, c->yc, c->div, c->partials,
+                    c->counter, c->obs_partials, c->ck, c->adj, c->grad};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->own) (void)hipStreamDestroy(c->own);
@@ -232,6 +256,69 @@ int import_device(tsmi_ctx *c, void *dst, const double *src) {
     return TSMI_OK;
 }
 
+// The periodic halo refresh of a state (or E) allocation: the reference's
+// 4-wide exchange on the reference tile, or on the 8-wide layout the full wrap
+// of all 8 halo cells, whose inner 4 are exactly what the 4-wide exchange writes.
+template <typename T>
+int exchange_halo(tsmi_ctx *c, void *base_re, void *base_im) {
+    if (!c->d.periodic_x && !c->d.periodic_y) return TSMI_OK;
+    const int W = c->halo8 ? c->Wd : c->W, H = c->halo8 ? c->Hd : c->H;
+    const int hx = c->halo8 ? 2 * c->d.halo_x : c->d.halo_x, hy = c->halo8 ? 2 * c->d.halo_y : c->d.halo_y;
+    const long cells = 2L * hy * W + 2L * hx * H;
+    const int eb = (int)std::min<long>(1024, (cells + 255) / 256);
+    hipLaunchKernelGGL((halo_exchange_kernel<T>), dim3(eb), dim3(256), 0, c->stream, (T *)base_re, (T *)base_im, W, H,
+                       c->pitch, c->d.periodic_x, c->d.periodic_y, hx, hy);
+    TSMI_HIP(hipGetLastError());
+    return TSMI_OK;
+}
+
+// Two real-time steps in one pass over the whole allocation (fused_kernel.hpp).
+template <typename T, int VEC, bool LHY, int MODE>
+int launch_step2(tsmi_ctx *c) {
+    StepParams<T> p;
+    std::memset(&p, 0, sizeof(p));
+    const int nb = 1 - c->cur;
+    p.in_re = (const T *)c->psi_base[c->cur][0];
+    p.in_im = (const T *)c->psi_base[c->cur][1];
+    p.out_re = (T *)c->psi_base[nb][0];
+    p.out_im = (T *)c->psi_base[nb][1];
+    p.e_re = (const T *)(c->halo8 ? c->e8[0] : c->e_base[0]);
+    p.e_im = (const T *)(c->halo8 ? c->e8[1] : c->e_base[1]);
+    p.W = c->Wd; p.H = c->Hd; p.pitch = c->pitch;
+    p.n_strips = c->n_strips2; p.n_waves = c->n_waves2; p.seg_rows = c->seg_rows2;
+    p.mH = (T)c->mH; p.tH = (T)c->tH; p.mV = (T)c->mV; p.tV = (T)c->tV;
+    p.unH = (T)c->unH; p.unV = (T)c->unV;
+    p.c_nl = (T)c->c_nl_s; p.c_lhy = (T)c->c_lhy_s;
+    const int blocks = (c->n_waves2 + kBlock / kWave - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL((step2_kernel<T, VEC, LHY, MODE>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
+    TSMI_HIP(hipGetLastError());
+    c->cur = nb;
+    return TSMI_OK;
+}
+
+template <typename T, int VEC, bool LHY>
+int step2_mode(tsmi_ctx *c) {
+    if (c->mode == 0) return launch_step2<T, VEC, LHY, 0>(c);
+    if (c->mode == 1) return launch_step2<T, VEC, LHY, 1>(c);
+    return launch_step2<T, VEC, LHY, -1>(c);
+}
+
+int step2_once(tsmi_ctx *c) {
+    const bool lhy = c->c_lhy != 0.0;
+    if (c->esz == 4) return lhy ? step2_mode<float, kVec32F, true>(c) : step2_mode<float, kVec32F, false>(c);
+    return lhy ? step2_mode<double, 2, true>(c) : step2_mode<double, 2, false>(c);
+}
+
+template <typename T, int VEC, bool LHY>
+hipError_t occupancy2_of(int *blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, step2_kernel<T, VEC, LHY, 0>, kBlock, 0);
+}
+
+hipError_t occupancy2_blocks(int precision, bool lhy, int *blocks) {
+    if (precision == 32) return lhy ? occupancy2_of<float, kVec32F, true>(blocks) : occupancy2_of<float, kVec32F, false>(blocks);
+    return lhy ? occupancy2_of<double, 2, true>(blocks) : occupancy2_of<double, 2, false>(blocks);
+}
+
 template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
 int launch_step(tsmi_ctx *c, int exchange) {
     StepParams<T> p;
@@ -261,15 +348,11 @@ int launch_step(tsmi_ctx *c, int exchange) {
     const int blocks = (c->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
     hipLaunchKernelGGL((step_kernel<T, IMAG, VEC, LHY, MODE>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
     TSMI_HIP(hipGetLastError());
-    if (exchange && (c->d.periodic_x || c->d.periodic_y)) {
-        const long cells = 2L * c->d.halo_y * c->W + 2L * c->d.halo_x * c->H;
-        const int eb = (int)std::min<long>(1024, (cells + 255) / 256);
-        hipLaunchKernelGGL((halo_exchange_kernel<T>), dim3(eb), dim3(256), 0, c->stream, (T *)c->psi_re[nb],
-                           (T *)c->psi_im[nb], c->W, c->H, c->pitch, c->d.periodic_x, c->d.periodic_y,
-                           c->d.halo_x, c->d.halo_y);
-        TSMI_HIP(hipGetLastError());
-    }
     c->cur = nb;
+    if (exchange) {
+        const int st = exchange_halo<T>(c, c->psi_base[nb][0], c->psi_base[nb][1]);
+        if (st) return st;
+    }
     if (IMAG) {
         c->div_slot = 1 - c->div_slot;
         c->div_pending = true;
@@ -284,8 +367,8 @@ hipError_t occupancy_of(int *blocks) {
 
 hipError_t occupancy_blocks(int precision, bool imag, bool lhy, int *blocks) {
     if (precision == 32) {
-        if (imag) return lhy ? occupancy_of<float, true, 4, true>(blocks) : occupancy_of<float, true, 4, false>(blocks);
-        return lhy ? occupancy_of<float, false, 4, true>(blocks) : occupancy_of<float, false, 4, false>(blocks);
+        if (imag) return lhy ? occupancy_of<float, true, kVec32, true>(blocks) : occupancy_of<float, true, kVec32, false>(blocks);
+        return lhy ? occupancy_of<float, false, kVec32, true>(blocks) : occupancy_of<float, false, kVec32, false>(blocks);
     }
     if (imag) return lhy ? occupancy_of<double, true, 2, true>(blocks) : occupancy_of<double, true, 2, false>(blocks);
     return lhy ? occupancy_of<double, false, 2, true>(blocks) : occupancy_of<double, false, 2, false>(blocks);
@@ -312,7 +395,50 @@ int step_typed(tsmi_ctx *c, int exchange) {
 }
 
 int step_once(tsmi_ctx *c, int exchange) {
-    return c->esz == 4 ? step_typed<float, 4>(c, exchange) : step_typed<double, 2>(c, exchange);
+    return c->esz == 4 ? step_typed<float, kVec32>(c, exchange) : step_typed<double, 2>(c, exchange);
+}
+
+int exchange_planes(tsmi_ctx *c, void *re, void *im) {
+    return c->esz == 4 ? exchange_halo<float>(c, re, im) : exchange_halo<double>(c, re, im);
+}
+
+// Solver::evolve(n) for one component (src/solver.cpp:186-230): n steps, the
+// halo exchange after all but the last.  With two steps per pass, on closed
+// axes the pairs simply replace single steps; on periodic axes (8-wide halo
+// layout) the first step runs alone on the reference tile (its halo is whatever
+// the caller or the previous call left, as in the reference), every pass starts
+// from a freshly wrapped halo, and the last step runs alone so that the halo it
+// leaves is the reference's unrefreshed one.
+int evolve_steps(tsmi_ctx *c, int n) {
+    int st = TSMI_OK;
+    if (c->sweeps != 2 || n < 2 || (c->halo8 && n < 3)) {
+        for (int i = 0; i < n && !st; ++i) st = step_once(c, i != n - 1 ? 1 : 0);
+        return st;
+    }
+    if (!c->halo8) {
+        for (int i = 0; i + 1 < n && !st; i += 2) st = step2_once(c);
+        if (!st && (n & 1)) st = step_once(c, 0);
+        return st;
+    }
+    st = step_once(c, 1);
+    int m = n - 2;
+    for (; m >= 2 && !st; m -= 2) {
+        st = step2_once(c);
+        if (!st) st = exchange_planes(c, c->psi_base[c->cur][0], c->psi_base[c->cur][1]);
+    }
+    if (!st && m == 1) st = step_once(c, 1);
+    if (!st) st = step_once(c, 0);
+    return st;
+}
+
+// The fused passes read E over the whole 8-wide halo: the reference-tile table
+// with every halo cell replaced by its wrapped inner cell.
+int wrap_exp(tsmi_ctx *c) {
+    if (!c->halo8) return TSMI_OK;
+    const size_t pb = plane_bytes(c);
+    TSMI_HIP(hipMemcpyAsync(c->e8[0], c->e_base[0], pb, hipMemcpyDeviceToDevice, c->stream));
+    TSMI_HIP(hipMemcpyAsync(c->e8[1], c->e_base[1], pb, hipMemcpyDeviceToDevice, c->stream));
+    return exchange_planes(c, c->e8[0], c->e8[1]);
 }
 
 // One reverse step: (s_in, g_in) -> (s_out, g_out), gradient into c->grad.
@@ -349,7 +475,7 @@ int adjoint_step(tsmi_ctx *c, const void *s_re, const void *s_im, const void *g_
 // adjoint of the initial state in the returned planes.
 int reverse_sweep(tsmi_ctx *c, const double *g_re, const double *g_im, void **g0_re, void **g0_im) {
     const size_t pb = plane_bytes(c);
-    const size_t pd = (size_t)c->pitch * (size_t)c->H * sizeof(double);
+    const size_t pd = (size_t)c->pitch * (size_t)c->Hd * sizeof(double);
     if (!c->adj) TSMI_HIP(hipMalloc(&c->adj, 8 * pb));
     if (!c->grad) TSMI_HIP(hipMalloc((void **)&c->grad, pd));
     char *a = (char *)c->adj;
@@ -453,13 +579,27 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     if (int st = check_desc(d, "tsmi_create")) return st;
     const long long W = (long long)d.dim_x + (d.periodic_x ? 2 * d.halo_x : 0);
     const long long H = (long long)d.dim_y + (d.periodic_y ? 2 * d.halo_y : 0);
-    const long long pitch = (W + 63) / 64 * 64;
+    if (d.sweeps < 0 || d.sweeps > 2) return fail(TSMI_E_INVALID, "tsmi_create: sweeps must be 0, 1 or 2");
+    // two steps per pass: real time, both axes closed or both periodic
+    const bool fusable = !d.imag_time && ((d.periodic_x && d.periodic_y) || (!d.periodic_x && !d.periodic_y));
+    if (d.sweeps == 2 && !fusable)
+        return fail(TSMI_E_UNSUPPORTED,
+                    "tsmi_create: two steps per pass need real time and both axes closed or both periodic");
+    const int sweeps = (d.sweeps == 1 || !fusable) ? 1 : 2;
+    const bool halo8 = sweeps == 2 && d.periodic_x && d.periodic_y;
+    const long long Wd = W + (halo8 ? 2 * d.halo_x : 0), Hd = H + (halo8 ? 2 * d.halo_y : 0);
+    const long long pitch = (Wd + 63) / 64 * 64;
 
     tsmi_ctx *c = new tsmi_ctx();
     c->d = d;
     c->W = (int)W; c->H = (int)H; c->pitch = (int)pitch;
+    c->Wd = (int)Wd; c->Hd = (int)Hd;
+    c->halo8 = halo8;
+    c->off_el = halo8 ? (size_t)d.halo_y * (size_t)pitch + (size_t)d.halo_x : 0;
+    c->sweeps = sweeps;
     c->esz = d.precision == 32 ? 4 : 8;
-    c->vec = c->esz == 4 ? 4 : 2;
+    c->vec = c->esz == 4 ? kVec32 : 2;
+    const int vec2 = c->esz == 4 ? kVec32F : 2;
     const int sw = kWave * c->vec - 2 * kApron;
     c->n_strips = (c->W + sw - 1) / sw;
     int seg = d.tile_rows;
@@ -485,6 +625,27 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     c->n_waves = c->n_strips * c->n_segs;
     c->ix0 = d.periodic_x ? d.halo_x : 0; c->ix1 = c->W - c->ix0;
     c->iy0 = d.periodic_y ? d.halo_y : 0; c->iy1 = c->H - c->iy0;
+    if (sweeps == 2) {
+        // fused passes: strips of kWave*vec - 16 columns over the whole allocation,
+        // segments sized to one round of resident waves of the fused kernel
+        const int sw2 = kWave * vec2 - 4 * kApron;
+        c->n_strips2 = (c->Wd + sw2 - 1) / sw2;
+        int seg2 = std::max<int>(16, (int)((Hd + 1) / 2));
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            occupancy2_blocks(d.precision, d.coupling_lhy != 0.0, &per_cu) == hipSuccess && cus > 0 && per_cu > 0) {
+            const long resident = (long)cus * per_cu * (kBlock / kWave);
+            const long segs = std::max<long>(1, resident / c->n_strips2);
+            seg2 = (int)std::max<long>(16, (Hd + segs - 1) / segs);
+        }
+        if (d.tile_rows > 0) seg2 = d.tile_rows;
+        seg2 = std::min(seg2, max_seg_rows(pitch));
+        seg2 = (seg2 + 1) / 2 * 2;
+        c->seg_rows2 = std::min<int>(seg2, c->Hd);
+        c->n_segs2 = (c->Hd + c->seg_rows2 - 1) / c->seg_rows2;
+        c->n_waves2 = c->n_strips2 * c->n_segs2;
+    }
 
     // kinetic and nonlinear coefficients in FP64 (src/cpukernel.cpp:195-216) and
     // the kernel's scaled-frame factors
@@ -510,21 +671,31 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     if (e != hipSuccess) return bail(TSMI_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     c->stream = c->own;
     const size_t pb = plane_bytes(c);
-    const size_t pd = (size_t)c->pitch * (size_t)c->H * sizeof(double);
+    const size_t pd = (size_t)c->pitch * (size_t)c->Hd * sizeof(double);
     c->obs_blocks = std::min(kObsBlocksMax, c->H);
     struct { void **p; size_t n; } allocs[] = {
-        {&c->psi_re[0], pb}, {&c->psi_im[0], pb}, {&c->psi_re[1], pb}, {&c->psi_im[1], pb},
-        {&c->e_re, pb}, {&c->e_im, pb}, {(void **)&c->v, pd},
+        {&c->psi_base[0][0], pb}, {&c->psi_base[0][1], pb}, {&c->psi_base[1][0], pb}, {&c->psi_base[1][1], pb},
+        {&c->e_base[0], pb}, {&c->e_base[1], pb}, {&c->e8[0], halo8 ? pb : 0}, {&c->e8[1], halo8 ? pb : 0},
+        {(void **)&c->v_base, pd},
         {(void **)&c->xc, (size_t)c->W * sizeof(double)}, {(void **)&c->yc, (size_t)c->H * sizeof(double)},
         {(void **)&c->div, 2 * sizeof(double)}, {(void **)&c->partials, (size_t)c->n_waves * sizeof(double)},
         {(void **)&c->counter, 64}, {(void **)&c->obs_partials, (size_t)c->obs_blocks * OBS_COUNT * sizeof(double)},
     };
     for (auto &a : allocs) {
+        if (a.n == 0) continue;
         e = hipMalloc(a.p, a.n);
         if (e != hipSuccess) return bail(TSMI_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
         e = hipMemsetAsync(*a.p, 0, a.n, c->stream);
         if (e != hipSuccess) return bail(TSMI_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
     }
+    const size_t ob = c->off_el * (size_t)c->esz;
+    for (int b = 0; b < 2; ++b) {
+        c->psi_re[b] = (char *)c->psi_base[b][0] + ob;
+        c->psi_im[b] = (char *)c->psi_base[b][1] + ob;
+    }
+    c->e_re = (char *)c->e_base[0] + ob;
+    c->e_im = (char *)c->e_base[1] + ob;
+    c->v = c->v_base + c->off_el;
     std::vector<double> xs, ys;
     map_axis(d.periodic_x ? -d.halo_x : 0, d.delta_x, d.dim_x, d.length_x, c->W, xs);
     map_axis(d.periodic_y ? -d.halo_y : 0, d.delta_y, d.dim_y, d.length_y, c->H, ys);
@@ -578,6 +749,7 @@ int tsmi_set_exp_potential_device(tsmi_ctx *c, const double *e_re, const double 
     if (!c->d.imag_time && !e_im) return fail(TSMI_E_INVALID, "tsmi_set_exp_potential_device: real time needs Im(E)");
     const int st = launch_import_exp(c, e_re, c->d.imag_time ? nullptr : e_im);
     if (st) return st;
+    if (int we = wrap_exp(c)) return we;
     c->has_e = true;
     return TSMI_OK;
 }
@@ -626,6 +798,7 @@ int tsmi_set_exp_potential(tsmi_ctx *c, const double *e_re, const double *e_im) 
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(TSMI_E_HIP, std::string("exp potential upload: ") + hipGetErrorString(e));
     if (st) return st;
+    if (int we = wrap_exp(c)) return we;
     c->has_e = true;
     return TSMI_OK;
 }
@@ -644,11 +817,7 @@ int tsmi_evolve(tsmi_ctx *c, int32_t n) {
     if (!c) return fail(TSMI_E_INVALID, "tsmi_evolve: null context");
     if (n < 0) return fail(TSMI_E_INVALID, "tsmi_evolve: negative step count");
     if (!c->has_state || !c->has_e) return fail(TSMI_E_INVALID, "tsmi_evolve: state and exp potential must be set");
-    for (int i = 0; i < n; ++i) {
-        const int st = step_once(c, i != n - 1 ? 1 : 0);
-        if (st) return st;
-    }
-    return TSMI_OK;
+    return evolve_steps(c, n);
 }
 
 int tsmi_step(tsmi_ctx *c, int32_t exchange) {
@@ -680,8 +849,8 @@ int tsmi_evolve_checkpointed(tsmi_ctx *c, int32_t n, int32_t every) {
     for (int i = 0; i < n; ++i) {
         if (i % every == 0) {
             char *dst = (char *)c->ck + (size_t)(i / every) * 2 * pb;
-            TSMI_HIP(hipMemcpyAsync(dst, c->psi_re[c->cur], pb, hipMemcpyDeviceToDevice, c->stream));
-            TSMI_HIP(hipMemcpyAsync(dst + pb, c->psi_im[c->cur], pb, hipMemcpyDeviceToDevice, c->stream));
+            TSMI_HIP(hipMemcpyAsync(dst, c->psi_base[c->cur][0], pb, hipMemcpyDeviceToDevice, c->stream));
+            TSMI_HIP(hipMemcpyAsync(dst + pb, c->psi_base[c->cur][1], pb, hipMemcpyDeviceToDevice, c->stream));
         }
         const int st = step_once(c, 0);
         if (st) return st;

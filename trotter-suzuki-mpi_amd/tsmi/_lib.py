@@ -32,7 +32,7 @@ class Desc(C.Structure):
                 ("periodic_x", C.c_int32), ("periodic_y", C.c_int32),
                 ("halo_x", C.c_int32), ("halo_y", C.c_int32),
                 ("imag_time", C.c_int32), ("precision", C.c_int32),
-                ("tile_rows", C.c_int32), ("reserved", C.c_int32),
+                ("tile_rows", C.c_int32), ("sweeps", C.c_int32),
                 ("delta_t", C.c_double), ("mass", C.c_double),
                 ("coupling", C.c_double), ("coupling_lhy", C.c_double),
                 ("delta_x", C.c_double), ("delta_y", C.c_double),

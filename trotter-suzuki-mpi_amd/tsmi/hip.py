@@ -29,14 +29,14 @@ class HipKernel:
 
     def __init__(self, lattice: Lattice2D, *, delta_t: float, mass: float = 1.0, coupling: float = 0.0,
                  coupling_lhy: float = 0.0, imag_time: bool = False, precision: int = 64,
-                 norm_target: float = 0.0, tile_rows: int = 0):
+                 norm_target: float = 0.0, tile_rows: int = 0, sweeps: int = 0):
         L = _lib.load()
         self.lattice = lattice
         self.delta_t = float(delta_t)
         self.imag_time = bool(imag_time)
         self.precision = int(precision)
         d = _lib.Desc(lattice.dim_x, lattice.dim_y, int(lattice.periodic_x), int(lattice.periodic_y),
-                      lattice.halo_x, lattice.halo_y, int(imag_time), int(precision), int(tile_rows), 0,
+                      lattice.halo_x, lattice.halo_y, int(imag_time), int(precision), int(tile_rows), int(sweeps),
                       float(delta_t), float(mass), float(coupling), float(coupling_lhy),
                       lattice.delta_x, lattice.delta_y, lattice.length_x, lattice.length_y, float(norm_target))
         h = C.c_void_p()
@@ -162,7 +162,7 @@ class HipKernel:
 
 
 def kernel_for(wl, precision: int = 64, norm_target: float | None = None, tile_rows: int = 0,
-               e_planes=None) -> HipKernel:
+               e_planes=None, sweeps: int = 0) -> HipKernel:
     """A context loaded with a workload's state, E table and raw potential."""
     lat = wl.lattice
     if norm_target is None:
@@ -171,7 +171,8 @@ def kernel_for(wl, precision: int = 64, norm_target: float | None = None, tile_r
             inner = wl.psi[lat.inner_y, lat.inner_x]
             norm_target = float(np.sum(inner.real ** 2 + inner.imag ** 2) * (lat.delta_x * lat.delta_y))
     k = HipKernel(lat, delta_t=wl.dt, mass=wl.mass, coupling=wl.g, coupling_lhy=wl.g_lhy,
-                  imag_time=wl.imag_time, precision=precision, norm_target=norm_target, tile_rows=tile_rows)
+                  imag_time=wl.imag_time, precision=precision, norm_target=norm_target, tile_rows=tile_rows,
+                  sweeps=sweeps)
     er, ei = e_planes if e_planes is not None else exp_potential(wl.v, wl.dt, wl.imag_time)
     k.set_exp_potential(er, ei)
     k.set_potential(wl.v)
