@@ -16,7 +16,9 @@ from golden_util import host_fingerprint, input_digest
 from tsmi import workloads as W
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+# evolution fixtures (adjoint_*.npz hold finite differences: tests/test_adjoint_oracle.py)
+FIXTURES = sorted(f for f in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                  if not os.path.basename(f).startswith("adjoint_"))
 
 
 def load(path):
