@@ -95,6 +95,15 @@ int tsmi_set_potential(tsmi_ctx *ctx, const double *v);
 /* The same, from device memory (W*H FP64, row stride W).  Asynchronous. */
 int tsmi_set_potential_device(tsmi_ctx *ctx, const double *v);
 
+/* Tabulates E from a raw potential V already on the device (W*H FP64, row
+ * stride W): E = exp(-i dt V) in real time, exp(-dt V) in imaginary time, as
+ * Solver::initialize_exp_potential computes it on the host (src/solver.cpp:74-109),
+ * and keeps V for tsmi_energy.  This is the time-dependent-potential update of
+ * Solver::evolve (src/solver.cpp:187-192: Potential::update, re-tabulation,
+ * ITrotterKernel::update_potential) with the table built on the device.
+ * Asynchronous on the context stream. */
+int tsmi_set_potential_exp_device(tsmi_ctx *ctx, const double *v);
+
 /* n_steps full steps, as the single-component loop of Solver::evolve
  * (src/solver.cpp:186-230): per step run_kernel_on_halo + run_kernel
  * (src/cpukernel.cpp:413-524), the periodic halo exchange except after the last

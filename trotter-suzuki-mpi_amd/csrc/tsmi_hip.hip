@@ -246,6 +246,31 @@ int launch_import_exp(tsmi_ctx *c, const double *re, const double *im) {
     return TSMI_OK;
 }
 
+// E = exp(-i dt V) (real time) or exp(-dt V) (imaginary time) of a raw FP64
+// potential W x H (row stride W), as Solver::initialize_exp_potential tabulates
+// it (src/solver.cpp:74-109), times the frame factor, rounded once; the raw V
+// goes to the pitched FP64 plane used by the energies.
+template <typename T>
+__global__ void exp_from_potential(const double *__restrict__ v, double dt, int imag, double cr, double ci,
+                                   T *__restrict__ dre, T *__restrict__ dim, double *__restrict__ vout, int W, int H,
+                                   int pitch) {
+    const long n = (long)W * H;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long y = i / W, x = i - y * W;
+        const double vi = v[i];
+        double er, ei;
+        if (imag) {
+            er = exp(-dt * vi);
+            ei = 0.0;
+        } else {
+            sincos(-dt * vi, &ei, &er);
+        }
+        dre[y * pitch + x] = (T)(er * cr - ei * ci);
+        dim[y * pitch + x] = (T)(er * ci + ei * cr);
+        vout[y * pitch + x] = vi;
+    }
+}
+
 int import_device(tsmi_ctx *c, void *dst, const double *src) {
     const int blocks = 2048;
     if (c->esz == 8)
@@ -751,6 +776,24 @@ int tsmi_set_exp_potential_device(tsmi_ctx *c, const double *e_re, const double 
     if (st) return st;
     if (int we = wrap_exp(c)) return we;
     c->has_e = true;
+    return TSMI_OK;
+}
+
+int tsmi_set_potential_exp_device(tsmi_ctx *c, const double *v) {
+    if (c) c->ck_valid = false;
+    if (!c || !v) return fail(TSMI_E_INVALID, "tsmi_set_potential_exp_device: null argument");
+    const double dt = c->d.delta_t;
+    const int imag = c->d.imag_time ? 1 : 0;
+    if (c->esz == 4)
+        hipLaunchKernelGGL(exp_from_potential<float>, dim3(2048), dim3(256), 0, c->stream, v, dt, imag, c->corr_re,
+                           c->corr_im, (float *)c->e_re, (float *)c->e_im, c->v, c->W, c->H, c->pitch);
+    else
+        hipLaunchKernelGGL(exp_from_potential<double>, dim3(2048), dim3(256), 0, c->stream, v, dt, imag, c->corr_re,
+                           c->corr_im, (double *)c->e_re, (double *)c->e_im, c->v, c->W, c->H, c->pitch);
+    TSMI_HIP(hipGetLastError());
+    if (int we = wrap_exp(c)) return we;
+    c->has_e = true;
+    c->has_v = true;
     return TSMI_OK;
 }
 

@@ -19,7 +19,7 @@ TSMI_OK = 0
 EXPORTS = (
     "tsmi_create", "tsmi_destroy", "tsmi_set_stream", "tsmi_set_state", "tsmi_set_state_device",
     "tsmi_set_exp_potential", "tsmi_set_exp_potential_device", "tsmi_set_potential",
-    "tsmi_set_potential_device", "tsmi_evolve", "tsmi_step", "tsmi_evolve_checkpointed", "tsmi_adjoint",
+    "tsmi_set_potential_device", "tsmi_set_potential_exp_device", "tsmi_evolve", "tsmi_step", "tsmi_evolve_checkpointed", "tsmi_adjoint",
     "tsmi_adjoint_device", "tsmi_batch_create", "tsmi_batch_destroy", "tsmi_batch_set_state",
     "tsmi_batch_set_exp_potential", "tsmi_batch_evolve", "tsmi_batch_get_state", "tsmi_batch_tune",
     "tsmi_batch_info", "tsmi_batch_synchronize", "tsmi_squared_norm", "tsmi_energy", "tsmi_expected_values",
@@ -66,6 +66,7 @@ def load():
     L.tsmi_set_state_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.tsmi_set_exp_potential_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.tsmi_set_potential_device.argtypes = [C.c_void_p, C.c_void_p]
+    L.tsmi_set_potential_exp_device.argtypes = [C.c_void_p, C.c_void_p]
     L.tsmi_get_state_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.tsmi_evolve.argtypes = [C.c_void_p, C.c_int32]
     L.tsmi_step.argtypes = [C.c_void_p, C.c_int32]

@@ -95,6 +95,15 @@ class HipKernel:
     def set_potential_device(self, v_ptr: int) -> None:
         _lib.check(self._L.tsmi_set_potential_device(self._h, C.c_void_p(v_ptr)), "tsmi_set_potential_device")
 
+    def set_potential_exp_device(self, v_ptr: int) -> None:
+        """E = exp(-i dt V) (or exp(-dt V)) tabulated on the device from a W*H
+        FP64 device array, which is also kept as the raw V for energy()."""
+        _lib.check(self._L.tsmi_set_potential_exp_device(self._h, C.c_void_p(v_ptr)), "tsmi_set_potential_exp_device")
+
+    def step(self, exchange: bool = True) -> None:
+        """One step; the periodic halo exchange after it when ``exchange``."""
+        _lib.check(self._L.tsmi_step(self._h, int(bool(exchange))), "tsmi_step")
+
     def get_state_device(self, re_ptr: int, im_ptr: int) -> None:
         _lib.check(self._L.tsmi_get_state_device(self._h, C.c_void_p(re_ptr), C.c_void_p(im_ptr)),
                    "tsmi_get_state_device")
