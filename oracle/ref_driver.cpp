@@ -17,12 +17,14 @@
 //   double expect[10]   = State: norm2, X, X^2, Y, Y^2, P_x, P_x^2, P_y, P_y^2, L_z
 // In time mode the driver runs n_warmup steps, then times evolve(n_steps) with
 // steady_clock and prints one JSON line; no output file is written.
+// `ref_driver case.bin --io prefix` exercises the text I/O instead (run_io).
 #include "trottersuzuki.h"
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -30,12 +32,56 @@
 
 static void die(const char *m) { std::fprintf(stderr, "ref_driver: %s\n", m); std::exit(2); }
 
+// Text I/O of the state and potential (src/common.cpp:161-191, src/model.cpp:254-322,
+// :342-380, :826-839): writes the state, its density and phase as text, reads the
+// state back with loadtxt (halo refill on periodic axes) and a potential from a text
+// file, and dumps the loaded tile and potential as doubles to <prefix>-loaded.bin.
+static int run_io(const char *case_path, const std::string &prefix) {
+    FILE *f = std::fopen(case_path, "rb");
+    if (!f) return 2;
+    int32_t hdr[10];
+    double par[6];
+    if (std::fread(hdr, 4, 10, f) != 10 || std::fread(par, 8, 6, f) != 6) return 2;
+    Lattice2D grid(hdr[2], par[0], hdr[3], par[1], hdr[4] != 0, hdr[5] != 0);
+    const size_t n = (size_t)grid.dim_x * (size_t)grid.dim_y;
+    std::vector<double> re(n), im(n), v(n);
+    if (std::fread(re.data(), 8, n, f) != n || std::fread(im.data(), 8, n, f) != n || std::fread(v.data(), 8, n, f) != n)
+        return 2;
+    std::fclose(f);
+    State state(&grid, 0, re.data(), im.data());
+    state.write_to_file(prefix + "-state");
+    state.write_particle_density(prefix);
+    state.write_phase(prefix);
+    {
+        FILE *pf = std::fopen((prefix + "-potential").c_str(), "w");
+        for (size_t k = 0; k < n; ++k) std::fprintf(pf, "%.17g%c", v[k], (k + 1) % grid.dim_x ? ' ' : '\n');
+        std::fclose(pf);
+    }
+    State loaded(&grid);
+    std::string sp = prefix + "-state";
+    loaded.loadtxt(const_cast<char *>(sp.c_str()));
+    std::string pp = prefix + "-potential";
+    Potential pot(&grid, const_cast<char *>(pp.c_str()));
+    FILE *o = std::fopen((prefix + "-loaded.bin").c_str(), "wb");
+    std::fwrite(loaded.p_real, 8, n, o);
+    std::fwrite(loaded.p_imag, 8, n, o);
+    for (int y = 0; y < grid.dim_y; ++y)
+        for (int x = 0; x < grid.dim_x; ++x) {
+            const double val = pot.get_value(x, y);
+            std::fwrite(&val, 8, 1, o);
+        }
+    std::fclose(o);
+    return 0;
+}
+
+
 template <class T> static void rd(FILE *f, T *p, size_t n) {
     if (std::fread(p, sizeof(T), n, f) != n) die("short read");
 }
 
 int main(int argc, char **argv) {
-    if (argc < 2) die("usage: ref_driver case.bin [out.bin]");
+    if (argc < 2) die("usage: ref_driver case.bin [out.bin | --io prefix]");
+    if (argc > 3 && std::strcmp(argv[2], "--io") == 0) return run_io(argv[1], argv[3]);
     FILE *f = std::fopen(argv[1], "rb");
     if (!f) die("cannot open case file");
     int32_t hdr[10];

@@ -4,8 +4,11 @@
 // per scenario; tests/test_cpp_api.py checks the values.
 //
 //   test_solver <kernel_type> [scenario]
+//   test_solver io case.bin prefix      (text I/O, no device work)
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 #include <stdexcept>
 #include <string>
 
@@ -104,7 +107,51 @@ static void errors() {
     std::printf("{\"scenario\": \"errors\", \"caught\": %d, \"expected\": 3}\n", caught);
 }
 
+
+// Text I/O of the state and potential (src/common.cpp:161-191, src/model.cpp:254-322,
+// :342-380, :826-839): writes the state, its density and phase as text, reads the
+// state back with loadtxt (halo refill on periodic axes) and a potential from a text
+// file, and dumps the loaded tile and potential as doubles to <prefix>-loaded.bin.
+static int run_io(const char *case_path, const std::string &prefix) {
+    FILE *f = std::fopen(case_path, "rb");
+    if (!f) return 2;
+    int32_t hdr[10];
+    double par[6];
+    if (std::fread(hdr, 4, 10, f) != 10 || std::fread(par, 8, 6, f) != 6) return 2;
+    Lattice2D grid(hdr[2], par[0], hdr[3], par[1], hdr[4] != 0, hdr[5] != 0);
+    const size_t n = (size_t)grid.dim_x * (size_t)grid.dim_y;
+    std::vector<double> re(n), im(n), v(n);
+    if (std::fread(re.data(), 8, n, f) != n || std::fread(im.data(), 8, n, f) != n || std::fread(v.data(), 8, n, f) != n)
+        return 2;
+    std::fclose(f);
+    State state(&grid, 0, re.data(), im.data());
+    state.write_to_file(prefix + "-state");
+    state.write_particle_density(prefix);
+    state.write_phase(prefix);
+    {
+        FILE *pf = std::fopen((prefix + "-potential").c_str(), "w");
+        for (size_t k = 0; k < n; ++k) std::fprintf(pf, "%.17g%c", v[k], (k + 1) % grid.dim_x ? ' ' : '\n');
+        std::fclose(pf);
+    }
+    State loaded(&grid);
+    std::string sp = prefix + "-state";
+    loaded.loadtxt(const_cast<char *>(sp.c_str()));
+    std::string pp = prefix + "-potential";
+    Potential pot(&grid, const_cast<char *>(pp.c_str()));
+    FILE *o = std::fopen((prefix + "-loaded.bin").c_str(), "wb");
+    std::fwrite(loaded.p_real, 8, n, o);
+    std::fwrite(loaded.p_imag, 8, n, o);
+    for (int y = 0; y < grid.dim_y; ++y)
+        for (int x = 0; x < grid.dim_x; ++x) {
+            const double val = pot.get_value(x, y);
+            std::fwrite(&val, 8, 1, o);
+        }
+    std::fclose(o);
+    return 0;
+}
+
 int main(int argc, char **argv) {
+    if (argc > 3 && std::strcmp(argv[1], "io") == 0) return run_io(argv[2], argv[3]);
     const std::string kt = argc > 1 ? argv[1] : "hip";
     const std::string only = argc > 2 ? argv[2] : "";
     struct { const char *name; void (*fn)(const std::string &); } all[] = {

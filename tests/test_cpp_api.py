@@ -88,3 +88,41 @@ def test_dropin_inside_reference(case):
     assert r["kernel"] == "HIP"
     assert r["rel_l2"] < 1e-12
     assert abs(r["kernel_norm"] - r["solver_norm"]) < 1e-12
+
+
+IO_CASES = [
+    dict(n_x=24, n_y=16, periodic_x=True, periodic_y=True),
+    dict(n_x=20, n_y=12, periodic_x=False, periodic_y=False),
+    dict(n_x=18, n_y=14, periodic_x=True, periodic_y=False),
+    dict(n_x=16, n_y=18, periodic_x=False, periodic_y=True),
+]
+
+
+@pytest.mark.parametrize("case", IO_CASES, ids=lambda c: f"{c['n_x']}x{c['n_y']}_p{int(c['periodic_x'])}{int(c['periodic_y'])}")
+def test_text_io_matches_reference(case):
+    """State::write_to_file / write_particle_density / write_phase, State::loadtxt
+    (with the halo refill of periodic axes) and Potential(grid, file), against the
+    reference library driven on the same tile: identical text, identical loaded
+    tiles.  No device work."""
+    from oracle import pyoracle
+    if not os.path.exists(TEST_SOLVER) or not pyoracle.ref_available("strict"):
+        pytest.skip("needs test_solver and the reference build (oracle/_ref)")
+    wl = W.small_case(steps=0, g=1.0, **case)
+    lat = wl.lattice
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "case.bin")
+        pyoracle.write_case(path, wl, 0)
+        ours, ref = os.path.join(td, "ours"), os.path.join(td, "ref")
+        assert subprocess.run([TEST_SOLVER, "io", path, ours], timeout=60).returncode == 0
+        assert subprocess.run([pyoracle.ref_driver("strict"), path, "--io", ref], timeout=60).returncode == 0
+        for suffix in ("-state", "-density", "-phase", "-potential"):
+            with open(ours + suffix) as a, open(ref + suffix) as b:
+                assert a.read() == b.read(), suffix
+        a = np.fromfile(ours + "-loaded.bin")
+        b = np.fromfile(ref + "-loaded.bin")
+        assert np.array_equal(a, b)
+    n = lat.W * lat.H
+    loaded = (a[:n] + 1j * a[n:2 * n]).reshape(lat.H, lat.W)
+    inner = (lat.inner_y, lat.inner_x)
+    np.testing.assert_allclose(loaded[inner], wl.psi[inner], rtol=1e-5, atol=1e-7)
+    assert np.array_equal(a[2 * n:].reshape(lat.H, lat.W), wl.v)

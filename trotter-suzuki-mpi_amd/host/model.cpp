@@ -277,13 +277,12 @@ TSMI_STATE_GETTER(get_mean_pypy, mean_PyPy)
 TSMI_STATE_GETTER(get_mean_angular_momentum, mean_angular_momentum)
 #undef TSMI_STATE_GETTER
 
-// Text output of the inner lattice, "(re,im) " per entry and one row per line
-// (src/common.cpp:161-170 via stamp, one process).
+// Text output of the inner lattice, "(re,im) " per entry and one row per line,
+// in the stream's default format (src/common.cpp:161-170 via stamp, one process).
 void State::write_to_file(string prefix) {
     std::ofstream out(prefix.c_str(), std::ios::out | std::ios::trunc);
     const int x0 = grid->inner_start_x - grid->start_x, x1 = grid->inner_end_x - grid->start_x;
     const int y0 = grid->inner_start_y - grid->start_y, y1 = grid->inner_end_y - grid->start_y;
-    out.precision(17);
     for (int y = y0; y < y1; ++y) {
         for (int x = x0; x < x1; ++x)
             out << "(" << p_real[(size_t)y * grid->dim_x + x] << "," << p_imag[(size_t)y * grid->dim_x + x] << ") ";
