@@ -610,7 +610,8 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     if (d.sweeps == 2 && !fusable)
         return fail(TSMI_E_UNSUPPORTED,
                     "tsmi_create: two steps per pass need real time and both axes closed or both periodic");
-    const int sweeps = (d.sweeps == 1 || !fusable) ? 1 : 2;
+    // 0 chooses: one step per pass (the fused pass is taken on request)
+    const int sweeps = (d.sweeps == 2 && fusable) ? 2 : 1;
     const bool halo8 = sweeps == 2 && d.periodic_x && d.periodic_y;
     const long long Wd = W + (halo8 ? 2 * d.halo_x : 0), Hd = H + (halo8 ? 2 * d.halo_y : 0);
     const long long pitch = (Wd + 63) / 64 * 64;
