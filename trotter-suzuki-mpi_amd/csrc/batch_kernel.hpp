@@ -2,14 +2,14 @@
 //
 // Each state of a batch keeps its own tile, E planes and scaled-frame factors
 // (one StepParams per state and buffer parity, in device memory).  A step is
-// one persistent launch: every resident wavefront takes work items -- a
-// (state, strip-segment) pair, the unit step_kernel gives one wave -- from a
-// queue counter until the queue is empty, so lattices of very different sizes
-// share the launch without a partial last round.  The items are sorted by
-// decreasing cost on the host, so the long ones start first.  Counters are used
-// in alternate launches: a launch draws from queue[parity] and clears
-// queue[1 - parity], which the previous launch used and the next one will, so
-// no host-side reset (and no memset node in a captured graph) is needed.
+// one persistent launch: the resident wavefronts walk the work items -- a
+// (state, strip-segment) pair, the unit step_kernel gives one wave -- with a
+// wave-uniform stride, so lattices of very different sizes share the launch
+// without a partial last round.  The items are sorted by decreasing cost on the
+// host, so the long ones are spread over all waves first.  The item index is
+// scalar arithmetic from the wave's own index: the loop's trip count is uniform
+// by construction, which keeps every lane of a wave on the same item through
+// the divergent parts of the step (the imaginary-time reduction).
 #pragma once
 #include "step_kernel.hpp"
 
@@ -17,15 +17,11 @@ namespace tsmi {
 
 template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
 __global__ void __launch_bounds__(256) batch_step_kernel(const StepParams<T> *__restrict__ params,
-                                                         const int2 *__restrict__ items, int n_items,
-                                                         unsigned *__restrict__ queue, int parity) {
+                                                         const int2 *__restrict__ items, int n_items) {
     const int lane = threadIdx.x & (kWave - 1);
-    if (blockIdx.x == 0 && threadIdx.x == 0) queue[1 - parity] = 0u;
-    for (;;) {
-        unsigned it = 0u;
-        if (lane == 0) it = atomicAdd(queue + parity, 1u);
-        it = __builtin_amdgcn_readfirstlane(it);
-        if (it >= (unsigned)n_items) break;   // every wave ends here once the queue is drained
+    const int first = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
+    const int stride = __builtin_amdgcn_readfirstlane(gridDim.x * (blockDim.x / kWave));
+    for (int it = first; it < n_items; it += stride) {
         const int2 e = items[it];
         step_wave<T, IMAG, VEC, LHY, MODE>(params[e.x], e.y, lane);
     }

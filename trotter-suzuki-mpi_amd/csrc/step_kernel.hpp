@@ -664,7 +664,8 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
             __threadfence();
             ticket = atomicAdd(p.counter, 1u);
         }
-        ticket = __shfl(ticket, 0, kWave);
+        // lane 0's ticket, as a wave-uniform value: the branch below is uniform
+        ticket = __builtin_amdgcn_readfirstlane(ticket);
         if (ticket == (unsigned)p.n_waves - 1) {
             __threadfence();
             double a2 = 0.0, c2 = 0.0;

@@ -68,7 +68,6 @@ struct tsmi_batch {
     void *params[2] = {nullptr, nullptr};
     int2 *items = nullptr;
     int n_items = 0, items_cap = 0;
-    unsigned *queue = nullptr;
     void *exch[2] = {nullptr, nullptr};
     int n_periodic = 0;
     int cur = 0;
@@ -89,7 +88,7 @@ void drop_graphs(tsmi_batch *b) {
 
 void free_batch(tsmi_batch *b) {
     drop_graphs(b);
-    void *ptrs[] = {b->arena, b->params[0], b->params[1], b->items, b->queue, b->exch[0], b->exch[1]};
+    void *ptrs[] = {b->arena, b->params[0], b->params[1], b->items, b->exch[0], b->exch[1]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->own) (void)hipStreamDestroy(b->own);
@@ -200,7 +199,7 @@ int upload_exchange(tsmi_batch *b) {
 template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
 int launch_batch_step(tsmi_batch *b, int parity) {
     hipLaunchKernelGGL((batch_step_kernel<T, IMAG, VEC, LHY, MODE>), dim3(b->grid), dim3(kBlock), 0, b->stream,
-                       (const StepParams<T> *)b->params[parity], (const int2 *)b->items, b->n_items, b->queue, parity);
+                       (const StepParams<T> *)b->params[parity], (const int2 *)b->items, b->n_items);
     TSMI_HIPB(hipGetLastError());
     return TSMI_OK;
 }
@@ -359,8 +358,6 @@ int tsmi_batch_create(const tsmi_desc *descs, int32_t n, tsmi_batch **out) {
     const size_t psz = b->esz == 4 ? sizeof(StepParams<float>) : sizeof(StepParams<double>);
     if (e == hipSuccess) e = hipMalloc(&b->params[0], (size_t)n * psz);
     if (e == hipSuccess) e = hipMalloc(&b->params[1], (size_t)n * psz);
-    if (e == hipSuccess) e = hipMalloc((void **)&b->queue, 256);
-    if (e == hipSuccess) e = hipMemsetAsync(b->queue, 0, 256, b->stream);
     if (e != hipSuccess) return bail(set_error(TSMI_E_OOM, std::string("batch allocation: ") + hipGetErrorString(e)));
     for (const Lattice &L : b->lat)
         if (int st = reset_divisor(b, L)) return bail(st);
@@ -533,12 +530,6 @@ int tsmi_batch_tune(tsmi_batch *b, int32_t steps, double *ms_per_step) {
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     restore();
-    // the trials left one queue counter non-zero; start the next call from zero
-    if (!st) {
-        const hipError_t ez = hipMemsetAsync(b->queue, 0, 256, b->stream);
-        if (ez == hipSuccess) (void)hipStreamSynchronize(b->stream);
-        else st = set_error(TSMI_E_HIP, std::string("tsmi_batch_tune: ") + hipGetErrorString(ez));
-    }
     if (st) return st;
     if (ms_per_step) *ms_per_step = best_all / steps;
     return TSMI_OK;
