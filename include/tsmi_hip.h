@@ -46,7 +46,7 @@ typedef struct {
     int32_t precision;             /* 64: FP64 throughout; 32: FP32 bulk, FP64 reductions */
     int32_t tile_rows;             /* rows per wavefront segment; 0 chooses              */
     int32_t sweeps;                /* steps per pass over HBM in tsmi_evolve: 0 chooses
-                                      (currently 1), 1, or 2 (real time with both axes
+                                      (2 where exact), 1, or 2 (real time with both axes
                                       closed or both periodic only)                      */
     double delta_t, mass;
     double coupling;               /* g (Hamiltonian::coupling_a)                        */

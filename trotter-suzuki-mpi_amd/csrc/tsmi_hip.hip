@@ -82,12 +82,15 @@ namespace {
 constexpr int kBlock = 256;          // 4 wavefronts per workgroup
 
 // FP32 columns per lane of the single-step and of the fused two-step kernel
-// (FP64 uses 2: one 16-byte piece per plane).  Compile-time tuning constants.
+// (FP64 uses 2: one 16-byte piece per plane).  Compile-time tuning constants,
+// set from measurements at 8192^2: the fused kernel with 4 columns per lane
+// needs more registers than one wave per SIMD allows without spills, with 2 it
+// runs three waves per SIMD.
 #ifndef TSMI_FP32_VEC
 #define TSMI_FP32_VEC 4
 #endif
 #ifndef TSMI_FP32_VEC2
-#define TSMI_FP32_VEC2 4
+#define TSMI_FP32_VEC2 2
 #endif
 constexpr int kVec32 = TSMI_FP32_VEC;
 constexpr int kVec32F = TSMI_FP32_VEC2;
@@ -610,8 +613,8 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     if (d.sweeps == 2 && !fusable)
         return fail(TSMI_E_UNSUPPORTED,
                     "tsmi_create: two steps per pass need real time and both axes closed or both periodic");
-    // 0 chooses: one step per pass (the fused pass is taken on request)
-    const int sweeps = (d.sweeps == 2 && fusable) ? 2 : 1;
+    // 0 chooses two steps per pass wherever they are exact
+    const int sweeps = (d.sweeps != 1 && fusable) ? 2 : 1;
     const bool halo8 = sweeps == 2 && d.periodic_x && d.periodic_y;
     const long long Wd = W + (halo8 ? 2 * d.halo_x : 0), Hd = H + (halo8 ? 2 * d.halo_y : 0);
     const long long pitch = (Wd + 63) / 64 * 64;

@@ -55,6 +55,15 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise TsmiError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    # One HIP runtime per process.  PyTorch ships its own libamdhip64.so.7 and
+    # this library links the system one under the same soname, so whichever is
+    # loaded first serves both.  Let PyTorch load its runtime first when it is
+    # installed: the reverse order hands PyTorch a runtime it was not built with
+    # and it then sees no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.tsmi_create.argtypes = [C.POINTER(Desc), C.POINTER(C.c_void_p)]
     L.tsmi_destroy.argtypes = [C.c_void_p]
