@@ -166,6 +166,14 @@ int tsmi_batch_tune(tsmi_batch *batch, int32_t steps, double *ms_per_step);
 int tsmi_batch_info(const tsmi_batch *batch, int32_t i, int32_t out[6]);
 int tsmi_batch_synchronize(tsmi_batch *batch);
 
+/* Kernel timing for measurement (no reference counterpart).  With timing on,
+ * every step-kernel launch is bracketed by HIP events on the context stream;
+ * tsmi_kernel_time syncs and reports out[0..3] = total ms and number of the
+ * fused two-step launches, total ms and number of the single-step launches
+ * since timing was (re)enabled. */
+int tsmi_kernel_timing(tsmi_ctx *ctx, int32_t enable);
+int tsmi_kernel_time(tsmi_ctx *ctx, double out[4]);
+
 /* Squared norm of the inner region times dx*dy:
  * ITrotterKernel::calculate_squared_norm (src/cpukernel.cpp:526-549).  Syncs. */
 int tsmi_squared_norm(tsmi_ctx *ctx, double *out);

@@ -114,6 +114,10 @@ struct tsmi_ctx {
     int Wd = 0, Hd = 0;
     size_t off_el = 0;                    // elements from allocation to view origin
     bool halo8 = false;
+    // kernel timing (tsmi_kernel_timing): one event pair per step-kernel launch
+    bool timing = false;
+    std::vector<hipEvent_t> ev[2][2];     // [fused?][start/end]
+    size_t ev_used[2] = {0, 0};
     // fused two-step passes (fused_kernel.hpp)
     int sweeps = 1;
     int n_strips2 = 0, n_segs2 = 0, seg_rows2 = 0, n_waves2 = 0;
@@ -169,6 +173,9 @@ int free_all(tsmi_ctx *c) {
                     c->counter, c->obs_partials, c->ck, c->adj, c->grad};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (auto &k : c->ev)
+        for (auto &v : k)
+            for (hipEvent_t e : v) (void)hipEventDestroy(e);
     if (c->own) (void)hipStreamDestroy(c->own);
     return TSMI_OK;
 }
@@ -284,6 +291,21 @@ int import_device(tsmi_ctx *c, void *dst, const double *src) {
     return TSMI_OK;
 }
 
+// Event pair around a step-kernel launch when timing is on (kind 1: fused).
+int time_mark(tsmi_ctx *c, int kind, int which) {
+    if (!c->timing) return TSMI_OK;
+    std::vector<hipEvent_t> &v = c->ev[kind][which];
+    const size_t i = c->ev_used[kind];
+    if (i >= v.size()) {
+        hipEvent_t e = nullptr;
+        TSMI_HIP(hipEventCreate(&e));
+        v.push_back(e);
+    }
+    TSMI_HIP(hipEventRecord(v[i], c->stream));
+    if (which == 1) ++c->ev_used[kind];
+    return TSMI_OK;
+}
+
 // The periodic halo refresh of a state (or E) allocation: the reference's
 // 4-wide exchange on the reference tile, or on the 8-wide layout the full wrap
 // of all 8 halo cells, whose inner 4 are exactly what the 4-wide exchange writes.
@@ -318,8 +340,10 @@ int launch_step2(tsmi_ctx *c) {
     p.unH = (T)c->unH; p.unV = (T)c->unV;
     p.c_nl = (T)c->c_nl_s; p.c_lhy = (T)c->c_lhy_s;
     const int blocks = (c->n_waves2 + kBlock / kWave - 1) / (kBlock / kWave);
+    if (int st = time_mark(c, 1, 0)) return st;
     hipLaunchKernelGGL((step2_kernel<T, VEC, LHY, MODE>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
     TSMI_HIP(hipGetLastError());
+    if (int st = time_mark(c, 1, 1)) return st;
     c->cur = nb;
     return TSMI_OK;
 }
@@ -374,8 +398,10 @@ int launch_step(tsmi_ctx *c, int exchange) {
     }
     // the grid covers every (strip, segment) pair exactly once
     const int blocks = (c->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
+    if (int st = time_mark(c, 0, 0)) return st;
     hipLaunchKernelGGL((step_kernel<T, IMAG, VEC, LHY, MODE>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
     TSMI_HIP(hipGetLastError());
+    if (int st = time_mark(c, 0, 1)) return st;
     c->cur = nb;
     if (exchange) {
         const int st = exchange_halo<T>(c, c->psi_base[nb][0], c->psi_base[nb][1]);
@@ -943,6 +969,29 @@ int tsmi_adjoint(tsmi_ctx *c, const double *g_re, const double *g_im, double *gr
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
     return st;
+}
+
+int tsmi_kernel_timing(tsmi_ctx *c, int32_t enable) {
+    if (!c) return fail(TSMI_E_INVALID, "tsmi_kernel_timing: null context");
+    c->timing = enable != 0;
+    c->ev_used[0] = c->ev_used[1] = 0;
+    return TSMI_OK;
+}
+
+int tsmi_kernel_time(tsmi_ctx *c, double out[4]) {
+    if (!c || !out) return fail(TSMI_E_INVALID, "tsmi_kernel_time: null argument");
+    TSMI_HIP(hipStreamSynchronize(c->stream));
+    for (int kind = 0; kind < 2; ++kind) {
+        double ms = 0.0;
+        for (size_t i = 0; i < c->ev_used[kind]; ++i) {
+            float t = 0.f;
+            TSMI_HIP(hipEventElapsedTime(&t, c->ev[kind][0][i], c->ev[kind][1][i]));
+            ms += t;
+        }
+        out[2 * (1 - kind)] = ms;
+        out[2 * (1 - kind) + 1] = (double)c->ev_used[kind];
+    }
+    return TSMI_OK;
 }
 
 int tsmi_squared_norm(tsmi_ctx *c, double *out) {

@@ -137,6 +137,16 @@ class HipKernel:
                                                C.c_void_p(grad_ptr), C.c_void_p(g0_re_ptr or 0),
                                                C.c_void_p(g0_im_ptr or 0)), "tsmi_adjoint_device")
 
+    def kernel_timing(self, enable: bool = True) -> None:
+        """Brackets every step-kernel launch with HIP events from now on."""
+        _lib.check(self._L.tsmi_kernel_timing(self._h, int(bool(enable))), "tsmi_kernel_timing")
+
+    def kernel_time(self) -> dict:
+        """Syncs; ms and launch counts of the fused and single-step kernels."""
+        out = np.zeros(4)
+        _lib.check(self._L.tsmi_kernel_time(self._h, _lib.dptr(out)), "tsmi_kernel_time")
+        return {"fused_ms": out[0], "fused_launches": int(out[1]), "single_ms": out[2], "single_launches": int(out[3])}
+
     def synchronize(self) -> None:
         _lib.check(self._L.tsmi_synchronize(self._h), "tsmi_synchronize")
 
