@@ -4,10 +4,11 @@
 // side (the dependency cone of two steps) and two register pipelines in
 // series: pipeline A runs the nine sub-passes of step n on the input rows, and
 // every row it emits (4 rows behind its input) is the input row of pipeline B,
-// which runs step n+1 and emits 8 rows behind the input.  Each state row and each
-// E row is read once from HBM and the state written once for two steps: pipeline
-// A's E rows are kept in a ring of 8 rows per wave in LDS, where pipeline B
-// reads them 4 rows later.
+// which runs step n+1 and emits 8 rows behind the input.  Each state row is read
+// once and written once for two steps; the E row is read by both pipelines, 4
+// rows apart.  (Keeping pipeline A's E rows in an LDS ring for pipeline B cut
+// the fetched bytes by a fifth but made the pass slower at 8192^2: the second
+// read is mostly served by the caches.)
 //
 // Two steps on a tile with closed edges equal two single steps exactly when no
 // halo exchange is due between them: on closed axes always; on periodic axes
@@ -82,11 +83,8 @@ __device__ __forceinline__ void pipe_beat_fast(Pipe<T, VEC> &q, Row<T, VEC> in, 
 // Two real-time steps of one wavefront's strip segment.  p.seg_rows, p.n_strips
 // and p.n_waves describe the fused decomposition (strips of kWave*VEC - 16
 // output columns).
-// LDS ring of E rows per wave: kRing rows x (re, im) x kWave*VEC values.
-constexpr int kRing = 8;
-
 template <typename T, int VEC, bool LHY, int MODE>
-__device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wave, const int lane, T *ring) {
+__device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wave, const int lane) {
     constexpr int AP = 2 * kApron;
     constexpr int SW = kWave * VEC - 2 * AP;
     const int strip = wave % p.n_strips;
@@ -126,7 +124,7 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
     auto valid = [&](int y) { return y >= y_lo && y < y_hi; };
 
     Pipe<T, VEC> A, B;
-    Row<T, VEC> inA, inB, eA, eB;
+    Row<T, VEC> inA, inB, eA, eB, fA, fB;
 #pragma unroll
     for (int k = 0; k < VEC; ++k) {
         A.p1.re[k] = A.p1.im[k] = A.p2.re[k] = A.p2.im[k] = T(0);
@@ -135,54 +133,31 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
         B.p3.re[k] = B.p3.im[k] = B.p4.re[k] = B.p4.im[k] = T(0);
         inA.re[k] = inA.im[k] = inB.re[k] = inB.im[k] = T(0);
         eA.re[k] = eA.im[k] = eB.re[k] = eB.im[k] = T(0);
+        fA.re[k] = fA.im[k] = fB.re[k] = fB.im[k] = T(0);
     }
-    // the ring starts finite (an invalid row's value must never be NaN: it meets
-    // valid rows with coefficient 0)
-    T *const ring_lane = ring + lane * VEC;
-    constexpr int kPlane = kWave * VEC;
-#pragma unroll
-    for (int r = 0; r < kRing; ++r)
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) ring_lane[(2 * r) * kPlane + k] = ring_lane[(2 * r + 1) * kPlane + k] = T(0);
-    auto ring_put = [&](int y, const Row<T, VEC> &e) {
-        T *q = ring_lane + 2 * (y & (kRing - 1)) * kPlane;
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) { q[k] = e.re[k]; q[kPlane + k] = e.im[k]; }
-    };
-    auto ring_get = [&](int y, Row<T, VEC> &e) {
-        const T *q = ring_lane + 2 * (y & (kRing - 1)) * kPlane;
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) { e.re[k] = q[k]; e.im[k] = q[kPlane + k]; }
-    };
-
     auto emit = [&](int y, const Row<T, VEC> &o) {
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
         buf_store<T, VEC>(r_out_re, v_st, so, o.re);
         buf_store<T, VEC>(r_out_im, v_st, so, o.im);
     };
 
-    // beat t: pipeline A takes input row t (E row t-2, then kept in the ring)
-    // and gives row t-4, the input of pipeline B (E row t-6, from the ring),
-    // which gives row t-8
-    auto beat = [&](auto tp, int t, const Row<T, VEC> &in, const Row<T, VEC> &ea) {
+    // beat t: pipeline A takes input row t (E row t-2) and gives row t-4, the
+    // input of pipeline B (E row t-6), which gives row t-8
+    auto beat = [&](auto tp, int t, const Row<T, VEC> &in, const Row<T, VEC> &ea, const Row<T, VEC> &eb) {
         constexpr int TP = decltype(tp)::value;
-        Row<T, VEC> a, b, eb;
-        ring_get(t - 6, eb);
+        Row<T, VEC> a, b;
         pipe_beat<TP, LHY>(A, in, ea, a, valid(t), valid(t - 1), valid(t - 2), valid(t - 3), valid(t - 4), hc, mV, tV,
                            unV, c, cl);
         pipe_beat<TP, LHY>(B, a, eb, b, valid(t - 4), valid(t - 5), valid(t - 6), valid(t - 7), valid(t - 8), hc, mV,
                            tV, unV, c, cl);
-        ring_put(t - 2, ea);
         const int y = t - 8;
         if (valid(y) && y >= y_out_lo && y < y_out_hi) emit(y, b);
     };
-    auto beat_fast = [&](auto tp, int t, const Row<T, VEC> &in, const Row<T, VEC> &ea) {
+    auto beat_fast = [&](auto tp, int t, const Row<T, VEC> &in, const Row<T, VEC> &ea, const Row<T, VEC> &eb) {
         constexpr int TP = decltype(tp)::value;
-        Row<T, VEC> a, b, eb;
-        ring_get(t - 6, eb);
+        Row<T, VEC> a, b;
         pipe_beat_fast<TP, LHY, MODE>(A, in, ea, a, kH, kV, c, cl);
         pipe_beat_fast<TP, LHY, MODE>(B, a, eb, b, kH, kV, c, cl);
-        ring_put(t - 2, ea);
         const int y = t - 8;
         if (y >= y_out_lo && y < y_out_hi) emit(y, b);
     };
@@ -195,31 +170,33 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
     const int t_end = y_hi + 8;
     for (; t < t_end; t += 2) {
         if (interior && t >= y_lo + 8 && t + 1 < y_hi) {
-            beat_fast(I0(), t, inA, eA);
+            beat_fast(I0(), t, inA, eA, fA);
             if (t + 2 < y_hi) load_row(r_in_re, r_in_im, t + 2, inA);
             load_row(r_e_re, r_e_im, t, eA);
-            beat_fast(I1(), t + 1, inB, eB);
+            load_row(r_e_re, r_e_im, t - 4, fA);
+            beat_fast(I1(), t + 1, inB, eB, fB);
             if (t + 3 < y_hi) load_row(r_in_re, r_in_im, t + 3, inB);
             load_row(r_e_re, r_e_im, t + 1, eB);
+            load_row(r_e_re, r_e_im, t - 3, fB);
         } else {
-            beat(I0(), t, inA, eA);
+            beat(I0(), t, inA, eA, fA);
             if (t + 2 < y_hi) load_row(r_in_re, r_in_im, t + 2, inA);
             if (t < y_hi) load_row(r_e_re, r_e_im, t, eA);
-            beat(I1(), t + 1, inB, eB);
+            if (valid(t - 4)) load_row(r_e_re, r_e_im, t - 4, fA);
+            beat(I1(), t + 1, inB, eB, fB);
             if (t + 3 < y_hi) load_row(r_in_re, r_in_im, t + 3, inB);
             if (t + 1 < y_hi) load_row(r_e_re, r_e_im, t + 1, eB);
+            if (valid(t - 3)) load_row(r_e_re, r_e_im, t - 3, fB);
         }
     }
 }
 
 template <typename T, int VEC, bool LHY, int MODE>
 __global__ void __launch_bounds__(256) step2_kernel(StepParams<T> p) {
-    __shared__ T ring[256 / kWave][kRing * 2 * kWave * VEC];
     const int lane = threadIdx.x & (kWave - 1);
-    const int local = threadIdx.x / kWave;
-    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + local);
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
-    step2_wave<T, VEC, LHY, MODE>(p, wave, lane, ring[local]);
+    step2_wave<T, VEC, LHY, MODE>(p, wave, lane);
 }
 
 }  // namespace tsmi

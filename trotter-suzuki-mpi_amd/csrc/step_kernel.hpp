@@ -139,10 +139,25 @@ __device__ __forceinline__ void fpair_q(T k, T pr, T pi, T &qr, T &qi) {
     } else { qr = fma(k, pr, qr); qi = fma(k, pi, qi); }
 }
 
-template <typename T>
-__device__ __forceinline__ T shfl_from_next(T v) { return __shfl_down(v, 1, kWave); }
-template <typename T>
-__device__ __forceinline__ T shfl_from_prev(T v) { return __shfl_up(v, 1, kWave); }
+// Neighbour-lane values through DPP whole-wave shifts by one lane (no LDS
+// traffic): lane i gets lane i+1's value (from_next) or lane i-1's (from_prev);
+// the lane without a neighbour gets 0, which only ever reaches apron columns.
+__device__ __forceinline__ int dpp_from_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_from_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ float shfl_from_next(float v) {
+    return __builtin_bit_cast(float, dpp_from_next(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float shfl_from_prev(float v) {
+    return __builtin_bit_cast(float, dpp_from_prev(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double shfl_from_next(double v) {
+    const int2 b = __builtin_bit_cast(int2, v);
+    return __builtin_bit_cast(double, make_int2(dpp_from_next(b.x), dpp_from_next(b.y)));
+}
+__device__ __forceinline__ double shfl_from_prev(double v) {
+    const int2 b = __builtin_bit_cast(int2, v);
+    return __builtin_bit_cast(double, make_int2(dpp_from_prev(b.x), dpp_from_prev(b.y)));
+}
 
 // Horizontal coefficients of this lane.  A pair that is not inside the tile (or
 // straddles the wave's edge) gets (1, 0), which leaves both cells unchanged.
