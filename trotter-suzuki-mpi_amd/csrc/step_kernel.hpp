@@ -641,30 +641,52 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         if (y >= y_out_lo && y < y_out_hi) emit(y, o4);
     };
 
-    // Software pipeline: rows t+1, t+2 and E rows t-1, t are in flight while
-    // row t runs.  A row that is not loaded (past the segment) keeps the finite
-    // values of an older row; every pair touching it has coefficients (1, 0).
+    // Software pipeline, four rows ahead: state rows t..t+3 and E rows t-2..t+1
+    // are in flight while row t runs (a latency-bound wave -- small lattices,
+    // short segments -- gains most).  A row that is not loaded (past the
+    // segment) keeps the finite values of an older row; every pair touching it
+    // has coefficients (1, 0).
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
     int t = y_lo;
+    const int t_end = y_hi + 4;
+    Row<T, VEC> inC, inD, eC, eD;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) inC.re[k] = inC.im[k] = inD.re[k] = inD.im[k] = eC.re[k] = eC.im[k] = eD.re[k] = eD.im[k] = T(0);
     if (t < y_hi) load_psi(t, inA);
     if (t + 1 < y_hi) load_psi(t + 1, inB);
-    const int t_end = y_hi + 4;
-    for (; t < t_end; t += 2) {
-        if (interior && t >= y_lo + 4 && t + 1 < y_hi) {
-            beat_fast(I0(), t, inA, eA);
-            if (t + 2 < y_hi) load_psi(t + 2, inA);
-            load_e(t, eA);
-            beat_fast(I1(), t + 1, inB, eB);
-            if (t + 3 < y_hi) load_psi(t + 3, inB);
-            load_e(t + 1, eB);
+    if (t + 2 < y_hi) load_psi(t + 2, inC);
+    if (t + 3 < y_hi) load_psi(t + 3, inD);
+    if (t < y_hi) load_e(t, eA);
+    if (t + 1 < y_hi) load_e(t + 1, eB);
+    for (; t < t_end; t += 4) {
+        // slots: state row r in in[(r - y_lo) % 4]; E row r in e[(r - y_lo) % 4]
+        if (interior && t >= y_lo + 4 && t + 3 < y_hi) {
+            beat_fast(I0(), t, inA, eC);
+            if (t + 4 < y_hi) load_psi(t + 4, inA);
+            load_e(t + 2, eC);
+            beat_fast(I1(), t + 1, inB, eD);
+            if (t + 5 < y_hi) load_psi(t + 5, inB);
+            load_e(t + 3, eD);
+            beat_fast(I0(), t + 2, inC, eA);
+            if (t + 6 < y_hi) load_psi(t + 6, inC);
+            if (t + 4 < y_hi) load_e(t + 4, eA);
+            beat_fast(I1(), t + 3, inD, eB);
+            if (t + 7 < y_hi) load_psi(t + 7, inD);
+            if (t + 5 < y_hi) load_e(t + 5, eB);
         } else {
-            beat(I0(), t, inA, eA);
-            if (t + 2 < y_hi) load_psi(t + 2, inA);
-            if (t < y_hi) load_e(t, eA);
-            beat(I1(), t + 1, inB, eB);
-            if (t + 3 < y_hi) load_psi(t + 3, inB);
-            if (t + 1 < y_hi) load_e(t + 1, eB);
+            beat(I0(), t, inA, eC);
+            if (t + 4 < y_hi) load_psi(t + 4, inA);
+            if (t + 2 < y_hi) load_e(t + 2, eC);
+            beat(I1(), t + 1, inB, eD);
+            if (t + 5 < y_hi) load_psi(t + 5, inB);
+            if (t + 3 < y_hi) load_e(t + 3, eD);
+            beat(I0(), t + 2, inC, eA);
+            if (t + 6 < y_hi) load_psi(t + 6, inC);
+            if (t + 4 < y_hi) load_e(t + 4, eA);
+            beat(I1(), t + 3, inD, eB);
+            if (t + 7 < y_hi) load_psi(t + 7, inD);
+            if (t + 5 < y_hi) load_e(t + 5, eB);
         }
     }
 

@@ -661,8 +661,10 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     if (seg <= 0) {
         // Size the launch to one round of resident wavefronts: as many row
         // segments per strip as the device holds at this kernel's occupancy, so
-        // no partial second round idles most of the GPU.
-        seg = std::max<int>(16, (int)((H + 1) / 2));
+        // no partial second round idles most of the GPU; on small lattices at
+        // least 24 rows, below which the 4-row aprons cost more than the extra
+        // waves bring (measured on 2048^2 and 1024^2).
+        seg = std::max<int>(24, (int)((H + 1) / 2));
         int dev = 0, cus = 0, blocks_per_cu = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
@@ -670,7 +672,7 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
             cus > 0 && blocks_per_cu > 0) {
             const long resident = (long)cus * blocks_per_cu * (kBlock / kWave);
             const long segs = std::max<long>(1, resident / c->n_strips);
-            seg = (int)std::max<long>(16, (H + segs - 1) / segs);
+            seg = (int)std::max<long>(24, (H + segs - 1) / segs);
         }
     }
     seg = std::min(seg, max_seg_rows(pitch));
