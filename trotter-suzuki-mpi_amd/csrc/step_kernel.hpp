@@ -745,7 +745,7 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
 // and one launch suffices.
 // Destinations: [0, hy) and [H-hy, H) full rows (periodic y), then the x bands
 // of the rows in between (periodic x).
-__device__ __forceinline__ int exchange_cells(int W, int H, int px, int py, int hx, int hy) {
+__host__ __device__ __forceinline__ int exchange_cells(int W, int H, int px, int py, int hx, int hy) {
     return (py ? 2 * hy * W : 0) + (px ? 2 * hx * (H - (py ? 2 * hy : 0)) : 0);
 }
 
