@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-studies", action="store_true",
                     help="skip the precision study, the batch and the adjoint measurements")
+    ap.add_argument("--only-study", choices=["precision_study", "batch", "adjoint"],
+                    help="only run that study and print its JSON object (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (for counter calibration) and exit")
     return ap.parse_args()
@@ -241,6 +243,10 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.only_study:
+        fn = {"precision_study": precision_study, "batch": batch_study, "adjoint": adjoint_study}[args.only_study]
+        print(json.dumps({args.only_study: fn()}), flush=True)
         return
     from tsmi import workloads_torch as WT
 

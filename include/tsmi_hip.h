@@ -142,17 +142,21 @@ int tsmi_adjoint_device(tsmi_ctx *ctx, const double *g_re, const double *g_im, d
 /* Batches: many independent lattices advanced together.  No reference
  * counterpart (the reference runs one Solver per lattice).  Each lattice has its
  * own descriptor (size, boundaries, dt, mass, spacing, couplings); precision,
- * time direction and whether g_LHY != 0 are common to the batch.  A step of the
- * batch is one persistent launch over all lattices (plus one halo-exchange
- * launch when some lattice is periodic) and gives each lattice exactly
- * tsmi_evolve's step.
+ * time direction and whether g_LHY != 0 are common to the batch.  Each lattice
+ * gets exactly tsmi_evolve's steps.  A lattice whose descriptor allows two steps
+ * per pass (sweeps 0 or 2, real time, both axes closed or both periodic) is
+ * advanced like a context with that setting; the others one step per pass.  A
+ * pass of the batch is one persistent launch over all lattices of its kind
+ * (plus one halo-exchange launch when some of them are periodic).
  *
  * tsmi_batch_evolve(n) records its launch sequence in a HIP graph the first time
- * n is used from a given buffer parity and replays the graph afterwards;
- * asynchronous.  tsmi_batch_tune times the batch step (`steps` steps per trial)
- * for candidate segment heights, one size class (tile shape) at a time, keeps
- * the fastest, restores the states and reports the tuned time per step.
- * tsmi_batch_info: out[0..5] as tsmi_geometry for lattice i.
+ * n is used from given buffer parities and replays the graph afterwards;
+ * asynchronous.  tsmi_batch_tune times the batch (`steps` steps per trial, at
+ * least 6 when some lattice takes two steps per pass) for candidate segment
+ * heights of each size class's dominant pass, one class (kind and tile shape) at
+ * a time, keeps the fastest, restores the states and reports the tuned time per
+ * step.  tsmi_batch_info: out[0..2] W, H, pitch of lattice i's tile, out[3..5]
+ * strips, segments and rows per segment of its dominant pass.
  * set_state / set_exp_potential / get_state take W*H FP64 host arrays of
  * lattice i; get_state syncs. */
 typedef struct tsmi_batch tsmi_batch;

@@ -210,3 +210,109 @@ def time_reference(wl, steps: int, n_warmup: int = 1, kind: str = "fast", thread
         write_case(cpath, wl, steps, n_warmup=n_warmup, time_mode=True)
         out = subprocess.run([ref_driver(kind), cpath], check=True, env=env, capture_output=True, text=True)
     return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+# ---------------------------------------------------------------------------
+# Two-component systems (PairWorkload)
+# ---------------------------------------------------------------------------
+
+class Desc2(C.Structure):
+    _fields_ = [("a", Desc), ("b", Desc), ("c_ab", C.c_double), ("omega_r", C.c_double), ("omega_i", C.c_double)]
+
+
+def _lib2():
+    L = lib()
+    if not getattr(L, "_pair_ready", False):
+        L.ts_oracle_rabi.argtypes = [C.POINTER(Desc2), C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.ts_oracle_evolve2.argtypes = [C.POINTER(Desc2), C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp,
+                                        _dp, _dp, _dp, _dp, _dp, C.c_double]
+        L.ts_oracle_energy2.argtypes = [C.POINTER(Desc2), _dp, _dp, _dp, _dp, _dp, _dp] + [C.c_double] * 8 + \
+            [C.c_int, _dp]
+        L._pair_ready = True
+    return L
+
+
+def make_desc2(pw) -> Desc2:
+    return Desc2(make_desc(pw.component(0), norm_target=0.0), make_desc(pw.component(1), norm_target=0.0),
+                 pw.g_ab * pw.dt, pw.omega_r, pw.omega_i)
+
+
+def _planes(psi):
+    return (np.ascontiguousarray(psi.real, dtype=np.float64).copy(),
+            np.ascontiguousarray(psi.imag, dtype=np.float64).copy())
+
+
+def pair_exp_potentials(pw):
+    return exp_potential(pw.v_a, pw.dt, pw.imag_time), exp_potential(pw.v_b, pw.dt, pw.imag_time)
+
+
+def evolve2(pw, calls=None, psi_a=None, psi_b=None, initial_half: bool = True):
+    """Oracle Solver::evolve for each entry of ``calls`` on one kernel (the
+    per-component norm targets persist between calls).  Returns (psi_a, psi_b)."""
+    calls = pw.calls if calls is None else calls
+    d = make_desc2(pw)
+    ar, ai = _planes(pw.psi_a if psi_a is None else psi_a)
+    br, bi = _planes(pw.psi_b if psi_b is None else psi_b)
+    L = _lib2()
+    norm = np.array([L.ts_oracle_state_norm(C.byref(d.a), _p(ar), _p(ai)),
+                     L.ts_oracle_state_norm(C.byref(d.b), _p(br), _p(bi))])
+    tot = float(norm[0] + norm[1])
+    (ear, eai), (ebr, ebi) = pair_exp_potentials(pw)
+    for n in calls:
+        L.ts_oracle_evolve2(C.byref(d), int(n), int(initial_half), pw.dt, _p(ar), _p(ai), _p(br), _p(bi),
+                            _p(ear), _p(eai), _p(ebr), _p(ebi), _p(norm), tot)
+    return ar + 1j * ai, br + 1j * bi
+
+
+def rabi(pw, var: float, psi_a, psi_b):
+    d = make_desc2(pw)
+    ar, ai = _planes(psi_a)
+    br, bi = _planes(psi_b)
+    _lib2().ts_oracle_rabi(C.byref(d), var, pw.dt, _p(ar), _p(ai), _p(br), _p(bi))
+    return ar + 1j * ai, br + 1j * bi
+
+
+def energy2(pw, psi_a, psi_b):
+    lat = pw.lattice
+    d = make_desc2(pw)
+    ar, ai = _planes(psi_a)
+    br, bi = _planes(psi_b)
+    va = np.ascontiguousarray(pw.v_a, dtype=np.float64)
+    vb = np.ascontiguousarray(pw.v_b, dtype=np.float64)
+    out = np.zeros(12)
+    _lib2().ts_oracle_energy2(C.byref(d), _p(ar), _p(ai), _p(br), _p(bi), _p(va), _p(vb), pw.mass_a, pw.mass_b,
+                              pw.g_a, pw.g_b, pw.g_ab, pw.omega_r, pw.omega_i, lat.length_x, lat.dim_x, _p(out))
+    return out
+
+
+def write_case_pair(path: str, pw) -> None:
+    lat = pw.lattice
+    calls = [int(c) for c in pw.calls]
+    hdr = np.array([0x54534D49, 2, lat.dim_x, lat.dim_y, int(lat.periodic_x), int(lat.periodic_y),
+                    len(calls), int(pw.imag_time), 0, 0], dtype=np.int32)
+    par = np.array([lat.length_x, lat.length_y, pw.dt, pw.mass_a, pw.mass_b, pw.g_a, pw.g_ab, pw.g_b,
+                    pw.omega_r, pw.omega_i], dtype=np.float64)
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(np.array(calls, dtype=np.int32).tobytes())
+        f.write(par.tobytes())
+        for a in (pw.psi_a.real, pw.psi_a.imag, pw.psi_b.real, pw.psi_b.imag, pw.v_a, pw.v_b):
+            f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+
+
+def run_reference_pair(pw, kind: str = "strict", threads: int | None = None):
+    """Runs the reference two-state Solver; returns (psi_a, psi_b, getters[14])."""
+    lat = pw.lattice
+    n = lat.W * lat.H
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    with tempfile.TemporaryDirectory() as td:
+        cpath, opath = os.path.join(td, "case.bin"), os.path.join(td, "out.bin")
+        write_case_pair(cpath, pw)
+        subprocess.run([ref_driver(kind), cpath, opath], check=True, env=env)
+        raw = np.fromfile(opath, dtype=np.float64)
+    sh = (lat.H, lat.W)
+    a = raw[:n].reshape(sh) + 1j * raw[n:2 * n].reshape(sh)
+    b = raw[2 * n:3 * n].reshape(sh) + 1j * raw[3 * n:4 * n].reshape(sh)
+    return a, b, raw[4 * n:4 * n + 14].copy()

@@ -18,6 +18,15 @@
 // In time mode the driver runs n_warmup steps, then times evolve(n_steps) with
 // steady_clock and prints one JSON line; no output file is written.
 // `ref_driver case.bin --io prefix` exercises the text I/O instead (run_io).
+//
+// Version 2 case files describe a two-component system (run_pair):
+//   int32  magic, version 2, Nx, Ny, periodic_x, periodic_y, n_calls, imag_time, 0, 0
+//   int32  steps[n_calls]                 one Solver::evolve call each
+//   double Lx, Ly, dt, mass, mass_b, g_a, g_ab, g_b, omega_r, omega_i
+//   double a_re, a_im, b_re, b_im, V_a, V_b   [W*H each]
+// Output: a_re, a_im, b_re, b_im [W*H each], then the Solver getters
+//   total, kinetic 1, 2, potential 1, 2, intra 1, 2, inter, Rabi, squared norm 1, 2, 3
+// and State::get_squared_norm of both states.
 #include "trottersuzuki.h"
 #include <chrono>
 #include <cstdint>
@@ -79,6 +88,56 @@ template <class T> static void rd(FILE *f, T *p, size_t n) {
     if (std::fread(p, sizeof(T), n, f) != n) die("short read");
 }
 
+// Two-component system through Hamiltonian2Component and the two-state Solver
+// (src/trottersuzuki.h:459-497, src/solver.cpp:43-60,150-242).
+static int run_pair(FILE *f, const int32_t *hdr, const char *out_path) {
+    const int Nx = hdr[2], Ny = hdr[3], px = hdr[4], py = hdr[5], n_calls = hdr[6], imag = hdr[7];
+    std::vector<int32_t> calls(n_calls > 0 ? n_calls : 0);
+    rd(f, calls.data(), calls.size());
+    double par[10];
+    rd(f, par, 10);
+    Lattice2D grid(Nx, par[0], Ny, par[1], px != 0, py != 0);
+    const size_t n = (size_t)grid.dim_x * (size_t)grid.dim_y;
+    std::vector<double> ar(n), ai(n), br(n), bi(n), va(n), vb(n);
+    rd(f, ar.data(), n);
+    rd(f, ai.data(), n);
+    rd(f, br.data(), n);
+    rd(f, bi.data(), n);
+    rd(f, va.data(), n);
+    rd(f, vb.data(), n);
+    std::fclose(f);
+    State sa(&grid, 0, ar.data(), ai.data());
+    State sb(&grid, 0, br.data(), bi.data());
+    Potential pa(&grid, va.data()), pb(&grid, vb.data());
+    Hamiltonian2Component ham(&grid, &pa, &pb, par[3], par[4], par[5], par[6], par[7], par[8], par[9]);
+    Solver solver(&grid, &sa, &sb, &ham, par[2], "cpu");
+    for (int32_t c : calls) solver.evolve(c, imag != 0);
+    double e[14];
+    e[0] = solver.get_total_energy();
+    e[1] = solver.get_kinetic_energy(1);
+    e[2] = solver.get_kinetic_energy(2);
+    e[3] = solver.get_potential_energy(1);
+    e[4] = solver.get_potential_energy(2);
+    e[5] = solver.get_intra_species_energy(1);
+    e[6] = solver.get_intra_species_energy(2);
+    e[7] = solver.get_inter_species_energy();
+    e[8] = solver.get_rabi_energy();
+    e[9] = solver.get_squared_norm(1);
+    e[10] = solver.get_squared_norm(2);
+    e[11] = solver.get_squared_norm(3);
+    e[12] = sa.get_squared_norm();
+    e[13] = sb.get_squared_norm();
+    FILE *o = std::fopen(out_path, "wb");
+    if (!o) die("cannot open output");
+    std::fwrite(ar.data(), 8, n, o);
+    std::fwrite(ai.data(), 8, n, o);
+    std::fwrite(br.data(), 8, n, o);
+    std::fwrite(bi.data(), 8, n, o);
+    std::fwrite(e, 8, 14, o);
+    std::fclose(o);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) die("usage: ref_driver case.bin [out.bin | --io prefix]");
     if (argc > 3 && std::strcmp(argv[2], "--io") == 0) return run_io(argv[1], argv[3]);
@@ -86,6 +145,10 @@ int main(int argc, char **argv) {
     if (!f) die("cannot open case file");
     int32_t hdr[10];
     rd(f, hdr, 10);
+    if (hdr[0] == 0x54534d49 && hdr[1] == 2) {
+        if (argc < 3) die("missing output path");
+        return run_pair(f, hdr, argv[2]);
+    }
     if (hdr[0] != 0x54534d49 || hdr[1] != 1) die("bad case header");
     const int Nx = hdr[2], Ny = hdr[3], px = hdr[4], py = hdr[5];
     const int n_steps = hdr[6], imag = hdr[7], n_warm = hdr[8], time_mode = hdr[9];

@@ -413,3 +413,159 @@ int ts_oracle_adjoint(const ts_oracle_desc *d, int n_steps, int every,
     free(ck); free(chi); free(st); free(gg);
     return 0;
 }
+
+/* ---------------------------------------------------------------------------
+ * Two-component systems
+ * ------------------------------------------------------------------------- */
+
+/* Potential pass of one component of two (src/cpucartesian.cpp:75-92 real
+ * time, :115-126 imaginary time): the phase sees the component's own density
+ * and the other component's density pb, taken from the start of the step. */
+static void pass_potential2(const ts_oracle_desc *d, double *re, double *im, const double *e_re,
+                            const double *e_im, const double *pb_re, const double *pb_im, double c_ab) {
+    const size_t n = (size_t)d->W * (size_t)d->H;
+    const double c = d->c_nl;
+    #pragma omp parallel for schedule(static) if (n >= PAR_CELLS)
+    for (long long k = 0; k < (long long)n; ++k) {
+        const double r = re[k], i = im[k];
+        const double n2 = r * r + i * i;
+        const double n2b = pb_re[k] * pb_re[k] + pb_im[k] * pb_im[k];
+        if (d->imag_time) {
+            const double f = exp(-1. * (c * n2 + c_ab * n2b));
+            re[k] = f * e_re[k] * r;
+            im[k] = f * e_re[k] * i;
+        } else {
+            const double ang = c * n2 + c_ab * n2b;
+            const double cs = cos(ang), sn = sin(ang);
+            const double r1 = e_re[k] * r - e_im[k] * i;
+            const double i1 = e_re[k] * i + e_im[k] * r;
+            re[k] = cs * r1 + sn * i1;
+            im[k] = cs * i1 - sn * r1;
+        }
+    }
+}
+
+/* full_step / full_step_imaginary with two wave functions (src/cpukernel.cpp:22-92). */
+static void full_step2(const ts_oracle_desc *d, double *re, double *im, const double *e_re, const double *e_im,
+                       const double *pb_re, const double *pb_im, double c_ab) {
+    const int vert = d->H > 1;
+    if (vert) ts_oracle_pass_vertical(d, 0, d->aV, d->bV, re, im);
+    ts_oracle_pass_horizontal(d, 0, d->aH, d->bH, re, im);
+    if (vert) ts_oracle_pass_vertical(d, 1, d->aV, d->bV, re, im);
+    ts_oracle_pass_horizontal(d, 1, d->aH, d->bH, re, im);
+    pass_potential2(d, re, im, e_re, e_im, pb_re, pb_im, c_ab);
+    ts_oracle_pass_horizontal(d, 1, d->aH, d->bH, re, im);
+    if (vert) ts_oracle_pass_vertical(d, 1, d->aV, d->bV, re, im);
+    ts_oracle_pass_horizontal(d, 0, d->aH, d->bH, re, im);
+    if (vert) ts_oracle_pass_vertical(d, 0, d->aV, d->bV, re, im);
+}
+
+void ts_oracle_rabi(const ts_oracle_desc2 *d, double var, double dt,
+                    double *a_re, double *a_im, double *b_re, double *b_im) {
+    /* src/cpukernel.cpp:581-608; the constants are half the Hamiltonian's omega (:330-331) */
+    const double cr = 0.5 * d->omega_r, ci = 0.5 * d->omega_i;
+    const double norm_omega = sqrt(cr * cr + ci * ci);
+    const int imag = d->a.imag_time;
+    double cc, cs_r, cs_i;
+    if (imag) {
+        cc = cosh(-dt * var * norm_omega);
+        cs_r = norm_omega == 0 ? 0 : cr / norm_omega * sinh(-dt * var * norm_omega);
+        cs_i = norm_omega == 0 ? 0 : ci / norm_omega * sinh(-dt * var * norm_omega);
+    } else {
+        cc = cos(-dt * var * norm_omega);
+        cs_r = norm_omega == 0 ? 0 : cr / norm_omega * sin(-dt * var * norm_omega);
+        cs_i = norm_omega == 0 ? 0 : ci / norm_omega * sin(-dt * var * norm_omega);
+    }
+    const size_t n = (size_t)d->a.W * (size_t)d->a.H;
+    for (size_t k = 0; k < n; ++k) {
+        const double r = a_re[k], i = a_im[k];
+        if (imag) {   /* src/cpucartesian.cpp:278-290 */
+            a_re[k] = cc * r + cs_r * b_re[k] - cs_i * b_im[k];
+            a_im[k] = cc * i + cs_i * b_re[k] + cs_r * b_im[k];
+            b_re[k] = cc * b_re[k] + cs_r * r + cs_i * i;
+            b_im[k] = cc * b_im[k] - cs_i * r + cs_r * i;
+        } else {      /* src/cpucartesian.cpp:264-276 */
+            a_re[k] = cc * r - cs_i * b_re[k] - cs_r * b_im[k];
+            a_im[k] = cc * i + cs_r * b_re[k] - cs_i * b_im[k];
+            b_re[k] = cc * b_re[k] + cs_i * r - cs_r * i;
+            b_im[k] = cc * b_im[k] + cs_r * r + cs_i * i;
+        }
+    }
+}
+
+static void scale_tile(size_t n, double f, double *re, double *im) {
+    for (size_t k = 0; k < n; ++k) {
+        re[k] /= f;
+        im[k] /= f;
+    }
+}
+
+void ts_oracle_evolve2(const ts_oracle_desc2 *d, int n_steps, int initial_half, double dt,
+                       double *a_re, double *a_im, double *b_re, double *b_im,
+                       const double *ea_re, const double *ea_im, const double *eb_re, const double *eb_im,
+                       double norm[2], double tot_norm) {
+    const size_t n = (size_t)d->a.W * (size_t)d->a.H;
+    const int imag = d->a.imag_time;
+    double *old_re = (double *)malloc(n * sizeof(double)), *old_im = (double *)malloc(n * sizeof(double));
+    if (!old_re || !old_im) {
+        free(old_re);
+        free(old_im);
+        return;
+    }
+    if (initial_half) ts_oracle_rabi(d, 0.5, dt, a_re, a_im, b_re, b_im);
+    for (int i = 0; i < n_steps; ++i) {
+        /* component b's step reads the start-of-step a (p[0][sense] in
+         * src/cpukernel.cpp:421-424 after wait_for_completion flipped sense back) */
+        memcpy(old_re, a_re, n * sizeof(double));
+        memcpy(old_im, a_im, n * sizeof(double));
+        full_step2(&d->a, a_re, a_im, ea_re, ea_im, b_re, b_im, d->c_ab);
+        if (i != n_steps - 1) ts_oracle_halo_exchange(&d->a, a_re, a_im);
+        if (imag && norm[0] != 0) scale_tile(n, sqrt(ts_oracle_squared_norm(&d->a, a_re, a_im) / norm[0]), a_re, a_im);
+        full_step2(&d->b, b_re, b_im, eb_re, eb_im, old_re, old_im, d->c_ab);
+        if (i != n_steps - 1) ts_oracle_halo_exchange(&d->b, b_re, b_im);
+        if (imag && norm[1] != 0) scale_tile(n, sqrt(ts_oracle_squared_norm(&d->b, b_re, b_im) / norm[1]), b_re, b_im);
+        ts_oracle_rabi(d, i == n_steps - 1 ? 0.5 : 1.0, dt, a_re, a_im, b_re, b_im);
+        if (imag && (d->omega_r != 0 || d->omega_i != 0)) {
+            /* src/cpukernel.cpp:610-667 */
+            const double sa = inner_sum(&d->a, a_re, a_im), sb = inner_sum(&d->b, b_re, b_im);
+            const double f = sqrt((sa + sb) * d->a.dx * d->a.dy / tot_norm);
+            scale_tile(n, f, a_re, a_im);
+            norm[0] = sa / (sa + sb) * tot_norm;
+            scale_tile(n, f, b_re, b_im);
+            norm[1] = sb / (sa + sb) * tot_norm;
+        }
+    }
+    free(old_re);
+    free(old_im);
+}
+
+void ts_oracle_energy2(const ts_oracle_desc2 *d, const double *a_re, const double *a_im,
+                       const double *b_re, const double *b_im, const double *va, const double *vb,
+                       double mass_a, double mass_b, double g_a, double g_b, double g_ab,
+                       double omega_r, double omega_i, double Lx, int Nx, double out[12]) {
+    double ea[6], eb[6];
+    ts_oracle_energy(&d->a, a_re, a_im, va, mass_a, g_a, 0.0, Lx, Nx, ea);
+    ts_oracle_energy(&d->b, b_re, b_im, vb, mass_b, g_b, 0.0, Lx, Nx, eb);
+    /* the cross sums and the plain norms (src/solver.cpp:340-354, :546-550) */
+    const size_t W = (size_t)d->a.W;
+    const region r = inner_region(&d->a);
+    const double complex om = CMPLX(omega_r, omega_i);
+    double s_a = 0, s_b = 0, s_ab = 0, s_rabi = 0;
+    for (int y = r.y0; y < r.y1; ++y)
+        for (int x = r.x0; x < r.x1; ++x) {
+            const size_t k = (size_t)y * W + (size_t)x;
+            const double complex pa = CMPLX(a_re[k], a_im[k]), pb = CMPLX(b_re[k], b_im[k]);
+            s_a += creal(conj(pa) * pa);
+            s_b += creal(conj(pb) * pb);
+            s_ab += creal(conj(pa) * pa * conj(pa) * pa * conj(pb) * pb * conj(pb) * pb * g_ab);
+            s_rabi += creal(conj(pa) * pa * conj(pb) * pb * (conj(pa) * pb * om + conj(pb * om) * pa));
+        }
+    const double inter = s_ab / (s_a * s_b), rabi = 0.5 * s_rabi / (s_a * s_b);
+    out[1] = ea[1]; out[2] = eb[1];
+    out[3] = ea[2]; out[4] = eb[2];
+    out[5] = ea[3]; out[6] = eb[3];
+    out[7] = inter; out[8] = rabi;
+    out[0] = ea[1] + ea[2] + ea[3] + eb[1] + eb[2] + eb[3] + inter + rabi;   /* src/solver.cpp:552-554 */
+    out[9] = ea[5]; out[10] = eb[5];
+    out[11] = ea[5] + eb[5];
+}

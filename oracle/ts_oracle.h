@@ -102,6 +102,57 @@ int ts_oracle_adjoint(const ts_oracle_desc *d, int n_steps, int every,
                       const double *g_re, const double *g_im,
                       double *grad_v, double *g0_re, double *g0_im);
 
+
+/* ---------------------------------------------------------------------------
+ * Two-component systems (Hamiltonian2Component, src/trottersuzuki.h:459-497,
+ * and the two-state CPUBlock, src/cpukernel.cpp:274-392).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    ts_oracle_desc a, b;     /* same lattice, time direction and spacings; each its own
+                                aH..bV (mass, mass_b) and c_nl = dt*g_a, dt*g_b
+                                (src/cpukernel.cpp:294-328); c_lhy must be 0 (the
+                                two-state potential pass has no LHY term, :332-335,
+                                src/cpucartesian.cpp:78-82)                         */
+    double c_ab;             /* dt*g_ab (src/cpukernel.cpp:329)                      */
+    double omega_r, omega_i; /* Hamiltonian2Component::omega_r/_i; the kernel uses
+                                half of each (src/cpukernel.cpp:330-331)             */
+} ts_oracle_desc2;
+
+/* CPUBlock::rabi_coupling(var, dt) (src/cpukernel.cpp:581-608) on the whole
+ * tiles, in place (src/cpucartesian.cpp:264-290).  Pinned: the closed-form
+ * population transfer cos^2(omega t / 2) and golden vectors. */
+void ts_oracle_rabi(const ts_oracle_desc2 *d, double var, double dt,
+                    double *a_re, double *a_im, double *b_re, double *b_im);
+
+/* Solver::evolve(n) of a two-component system on one kernel that persists
+ * between calls (src/solver.cpp:150-242): the half Rabi rotation first when
+ * initial_half != 0 (:173-177; the C++ API always, the Python one only at t = 0);
+ * per step component a's full step, whose potential pass also sees the other
+ * component's start-of-step density |psi_b|^2 (src/cpucartesian.cpp:75-92,
+ * :115-126), its halo exchange except after the last step, its imaginary-time
+ * renormalisation to norm[0] (src/cpukernel.cpp:551-570); component b likewise
+ * with the start-of-step |psi_a|^2 and norm[1]; the Rabi rotation by var = 1
+ * (0.5 after the last step, :222-225); in imaginary time with a Rabi coupling the joint
+ * renormalisation to tot_norm, which also redistributes norm[]
+ * (src/cpukernel.cpp:610-667).  norm[2] is the kernel's per-component target,
+ * in/out; tot_norm = norm[0] + norm[1] at kernel construction
+ * (src/cpukernel.cpp:321-325).
+ * Pinned: golden vectors, closed-form Rabi transfer, the reference's
+ * imaginary_mixed_BEC_test norms. */
+void ts_oracle_evolve2(const ts_oracle_desc2 *d, int n_steps, int initial_half, double dt,
+                       double *a_re, double *a_im, double *b_re, double *b_im,
+                       const double *ea_re, const double *ea_im, const double *eb_re, const double *eb_im,
+                       double norm[2], double tot_norm);
+
+/* Energies of a two-component system (src/solver.cpp:244-564, angular velocity
+ * 0).  va, vb are the raw potentials on the tile.  out[0..11] = total, kinetic
+ * a, kinetic b, potential a, potential b, intra a, intra b, inter-species, Rabi,
+ * squared norm a, squared norm b, total squared norm.  Pinned: golden vectors. */
+void ts_oracle_energy2(const ts_oracle_desc2 *d, const double *a_re, const double *a_im,
+                       const double *b_re, const double *b_im, const double *va, const double *vb,
+                       double mass_a, double mass_b, double g_a, double g_b, double g_ab,
+                       double omega_r, double omega_i, double Lx, int Nx, double out[12]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,3 +27,12 @@ def host_fingerprint() -> str:
         pass
     libc = os.confstr("CS_GNU_LIBC_VERSION") if hasattr(os, "confstr") else ""
     return f"{platform.machine()}|{libc}|{cpu}"
+
+
+def pair_digest(pw) -> str:
+    """SHA-256 of a two-component workload's input bytes (both initial tiles,
+    then both potentials)."""
+    h = hashlib.sha256()
+    for a in (pw.psi_a.real, pw.psi_a.imag, pw.psi_b.real, pw.psi_b.imag, pw.v_a, pw.v_b):
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return h.hexdigest()

@@ -16,9 +16,10 @@ from golden_util import host_fingerprint, input_digest
 from tsmi import workloads as W
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-# evolution fixtures (adjoint_*.npz hold finite differences: tests/test_adjoint_oracle.py)
+# single-component evolution fixtures (adjoint_*.npz hold finite differences:
+# tests/test_adjoint_oracle.py; pair*.npz two-component systems: tests/test_oracle_pair.py)
 FIXTURES = sorted(f for f in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(f).startswith("adjoint_"))
+                  if not os.path.basename(f).startswith(("adjoint_", "pair")))
 
 
 def load(path):

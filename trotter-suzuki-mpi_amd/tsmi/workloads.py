@@ -247,3 +247,72 @@ def small_case(n_x: int, n_y: int, periodic_x: bool, periodic_y: bool, *, imag_t
     v = 0.5 * harmonic(lat) + gaussian_sum(lat, seed, 2, offset=4)
     return Workload(f"small_{n_x}x{n_y}_p{int(periodic_x)}{int(periodic_y)}_{'imag' if imag_time else 'real'}",
                     lat, psi, v, dt, steps, imag_time=imag_time, g=g, g_lhy=g_lhy)
+
+
+@dataclass
+class PairWorkload:
+    """A two-component system on one lattice: Hamiltonian2Component
+    (src/trottersuzuki.h:459-497) with the two states of the two-state Solver
+    (src/solver.cpp:43-60).  ``calls`` are the step counts of consecutive
+    Solver::evolve calls."""
+    name: str
+    lattice: Lattice2D
+    psi_a: np.ndarray
+    psi_b: np.ndarray
+    v_a: np.ndarray
+    v_b: np.ndarray
+    dt: float
+    calls: tuple
+    imag_time: bool = False
+    mass_a: float = 1.0
+    mass_b: float = 1.0
+    g_a: float = 0.0
+    g_b: float = 0.0
+    g_ab: float = 0.0
+    omega_r: float = 0.0
+    omega_i: float = 0.0
+
+    @property
+    def steps(self) -> int:
+        return int(sum(self.calls))
+
+    def component(self, which: int) -> Workload:
+        """Single-component view of component 0 (a) or 1 (b): its state,
+        potential, mass and self coupling."""
+        if which == 0:
+            return Workload(self.name + "_a", self.lattice, self.psi_a, self.v_a, self.dt, self.steps,
+                            imag_time=self.imag_time, mass=self.mass_a, g=self.g_a)
+        return Workload(self.name + "_b", self.lattice, self.psi_b, self.v_b, self.dt, self.steps,
+                        imag_time=self.imag_time, mass=self.mass_b, g=self.g_b)
+
+
+def pair_case(n_x: int, n_y: int, periodic_x: bool, periodic_y: bool, *, imag_time: bool = False,
+              g_a: float = 0.0, g_b: float = 0.0, g_ab: float = 0.0, omega_r: float = 0.0,
+              omega_i: float = 0.0, mass_b: float = 1.0, calls=(10,), dt: float = 1e-3,
+              length: float = 20.0, seed: int = SEED_BASE + 500) -> PairWorkload:
+    """Small two-component parity case: two seeded packets (the second carrying a
+    third of the norm) in two different seeded potentials."""
+    lat = Lattice2D(n_x, length, n_y, length * n_y / n_x, periodic_x, periodic_y)
+    u = uniform(seed, 8, lo=-2.0, hi=2.0)
+    psi_a = normalise(lat, gaussian_packet(lat, u[0], u[1], u[2], u[3], width=1.5), 1.0)
+    psi_b = normalise(lat, gaussian_packet(lat, u[4], u[5], u[6], u[7], width=1.2), 1.0 / 3.0)
+    v_a = 0.5 * harmonic(lat) + gaussian_sum(lat, seed, 2, offset=8)
+    v_b = 0.4 * harmonic(lat) + gaussian_sum(lat, seed + 1, 2, offset=8)
+    tag = f"pair_{n_x}x{n_y}_p{int(periodic_x)}{int(periodic_y)}_{'imag' if imag_time else 'real'}"
+    return PairWorkload(tag, lat, psi_a, psi_b, v_a, v_b, dt, tuple(calls), imag_time=imag_time,
+                        mass_b=mass_b, g_a=g_a, g_b=g_b, g_ab=g_ab, omega_r=omega_r, omega_i=omega_i)
+
+
+def mixed_bec(imag_time: bool) -> PairWorkload:
+    """The two-component scenarios of the reference's kernel tests
+    (test/kerneltest.cpp:178-237): DIM 250, LENGTH 100, closed; component a a
+    Gaussian of omega 1, component b empty; both in the harmonic potential of
+    omega 1; no interactions; Rabi coupling omega_r = 2 pi / 10; dt = 1e-3;
+    5000 real-time or 1000 imaginary-time steps in one call."""
+    lat = Lattice2D(250, 100.0)
+    v = harmonic(lat)
+    psi_a = gaussian_state(lat, 1.0)
+    psi_b = np.zeros_like(psi_a)
+    steps = 1000 if imag_time else 5000
+    return PairWorkload("mixed_bec_" + ("imag" if imag_time else "real"), lat, psi_a, psi_b, v, v.copy(), 1e-3,
+                        (steps,), imag_time=imag_time, omega_r=2.0 * np.pi / 10.0)
