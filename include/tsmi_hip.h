@@ -170,6 +170,63 @@ int tsmi_batch_tune(tsmi_batch *batch, int32_t steps, double *ms_per_step);
 int tsmi_batch_info(const tsmi_batch *batch, int32_t i, int32_t out[6]);
 int tsmi_batch_synchronize(tsmi_batch *batch);
 
+/* Two-component systems: Hamiltonian2Component (src/trottersuzuki.h:459-497)
+ * evolved by the two-state CPUBlock (src/cpukernel.cpp:274-392) under the
+ * two-state Solver (src/solver.cpp:43-60,150-242).
+ *
+ * a and b describe the two components on one lattice: every lattice, time-step,
+ * time-direction, precision and tile_rows field identical; a.mass / b.mass =
+ * Hamiltonian2Component::mass / mass_b; a.coupling / b.coupling = coupling_a /
+ * coupling_b; coupling_lhy 0 (the two-state kernel has no LHY term,
+ * src/cpukernel.cpp:332-335); a.norm_target / b.norm_target = the squared norms
+ * of the two states when the kernel is built (src/solver.cpp:153-159), the
+ * imaginary-time targets.  sweeps is ignored (one step per pass). */
+typedef struct {
+    tsmi_desc a, b;
+    double coupling_ab;              /* Hamiltonian2Component::coupling_ab      */
+    double omega_r, omega_i;         /* Hamiltonian2Component::omega_r, omega_i */
+} tsmi_pair_desc;
+
+typedef struct tsmi_pair tsmi_pair;
+
+/* The two-state CPUBlock constructor (src/cpukernel.cpp:274-392). */
+int tsmi_pair_create(const tsmi_pair_desc *desc, tsmi_pair **out);
+void tsmi_pair_destroy(tsmi_pair *pair);
+
+/* Per component (which = 0: a, 1: b), W*H FP64 host arrays: the state
+ * (CPUBlock borrows state1/state2, src/cpukernel.cpp:364-367), E = exp(-i dt V)
+ * or exp(-dt V) (external_pot_real/imag[which], src/solver.cpp:74-109,
+ * ITrotterKernel::update_potential(..., which)), the raw potential for
+ * tsmi_pair_energy. */
+int tsmi_pair_set_state(tsmi_pair *pair, int32_t which, const double *re, const double *im);
+int tsmi_pair_set_exp_potential(tsmi_pair *pair, int32_t which, const double *e_re, const double *e_im);
+int tsmi_pair_set_potential(tsmi_pair *pair, int32_t which, const double *v);
+
+/* The two-component loop of Solver::evolve (src/solver.cpp:172-230): the half
+ * Rabi rotation first when initial_half != 0 (the C++ API always; the Python
+ * API only at t = 0, :174-175), then per step both components' steps (each
+ * potential pass sees the other component's start-of-step density), the halo
+ * exchange of both except after the last step, in imaginary time each
+ * component's renormalisation, the Rabi rotation by 1 (by 1/2 after the last
+ * step) and in imaginary time with a Rabi coupling the joint renormalisation
+ * (src/cpukernel.cpp:551-667).  The norm targets persist between calls, as in a
+ * kernel the Solver keeps.  Asynchronous. */
+int tsmi_pair_evolve(tsmi_pair *pair, int32_t n_steps, int32_t initial_half);
+
+/* get_sample of one component, the whole tile (src/cpukernel.cpp:572-579); syncs. */
+int tsmi_pair_get_state(tsmi_pair *pair, int32_t which, double *re, double *im);
+
+/* Solver::calculate_energy_expected_values for two components
+ * (src/solver.cpp:244-564, angular velocity 0): out[0..11] = total, kinetic a,
+ * kinetic b, potential a, potential b, intra a, intra b, inter-species, Rabi,
+ * squared norm a, squared norm b, total squared norm.  Needs both raw
+ * potentials; syncs. */
+int tsmi_pair_energy(tsmi_pair *pair, double out[12]);
+
+/* out[0..5] = W, H, pitch, strips, segments, rows per segment of the pair step. */
+int tsmi_pair_geometry(const tsmi_pair *pair, int32_t out[6]);
+int tsmi_pair_synchronize(tsmi_pair *pair);
+
 /* Kernel timing for measurement (no reference counterpart).  With timing on,
  * every step-kernel launch is bracketed by HIP events on the context stream;
  * tsmi_kernel_time syncs and reports out[0..3] = total ms and number of the

@@ -40,6 +40,21 @@ def test_invalid_descriptor_rejected_without_gpu():
     assert L.tsmi_evolve(None, 1) != 0
 
 
+def test_pair_descriptor_checks_without_gpu():
+    from tsmi import _lib
+    L = _lib.load()
+    h = C.c_void_p()
+    a = _lib.Desc(10, 10, 0, 0, 4, 4, 0, 64, 0, 0, 1e-3, 1.0, 0.0, 0.0, 0.1, 0.1, 1.0, 1.0, 0.0)
+    b = _lib.Desc(12, 10, 0, 0, 4, 4, 0, 64, 0, 0, 1e-3, 1.0, 0.0, 0.0, 0.1, 0.1, 1.0, 1.0, 0.0)
+    d = _lib.PairDesc(a, b, 0.0, 1.0, 0.0)
+    assert L.tsmi_pair_create(C.byref(d), C.byref(h)) == 1 and b"same lattice" in L.tsmi_last_error()
+    b = _lib.Desc(10, 10, 0, 0, 4, 4, 0, 64, 0, 0, 1e-3, 1.0, 0.0, 0.5, 0.1, 0.1, 1.0, 1.0, 0.0)
+    d = _lib.PairDesc(a, b, 0.0, 1.0, 0.0)
+    assert L.tsmi_pair_create(C.byref(d), C.byref(h)) == 4 and b"LHY" in L.tsmi_last_error()
+    assert L.tsmi_pair_evolve(None, 1, 1) != 0
+    assert L.tsmi_pair_set_state(None, 0, None, None) != 0
+
+
 def test_no_cpu_fallback_when_library_missing(monkeypatch, tmp_path):
     from tsmi import _lib
     monkeypatch.setattr(_lib, "_lib", None)

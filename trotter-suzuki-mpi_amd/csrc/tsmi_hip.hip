@@ -19,6 +19,7 @@
 
 #include "adjoint_kernel.hpp"
 #include "common.hpp"
+#include "context.hpp"
 #include "frame.hpp"
 #include "fused_kernel.hpp"
 #include "observables.hpp"
@@ -98,56 +99,6 @@ constexpr int kObsBlocksMax = 1024;
 
 }  // namespace
 
-struct tsmi_ctx {
-    tsmi_desc d;
-    int W = 0, H = 0, pitch = 0, esz = 0, vec = 0;
-    int n_strips = 0, n_segs = 0, seg_rows = 0, n_waves = 0;
-    // Views at the origin of the reference tile (W x H, row pitch `pitch`).  With
-    // the 8-wide halo layout they sit 4 rows and 4 columns inside allocations of
-    // Wd x Hd = (W + 8) x (H + 8); otherwise view and allocation coincide.
-    void *psi_re[2] = {nullptr, nullptr}, *psi_im[2] = {nullptr, nullptr};
-    void *e_re = nullptr, *e_im = nullptr;
-    void *psi_base[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    void *e_base[2] = {nullptr, nullptr};
-    void *e8[2] = {nullptr, nullptr};     // E wrapped over the whole 8-wide halo (fused passes)
-    double *v_base = nullptr;
-    int Wd = 0, Hd = 0;
-    size_t off_el = 0;                    // elements from allocation to view origin
-    bool halo8 = false;
-    // kernel timing (tsmi_kernel_timing): one event pair per step-kernel launch
-    bool timing = false;
-    std::vector<hipEvent_t> ev[2][2];     // [fused?][start/end]
-    size_t ev_used[2] = {0, 0};
-    // fused two-step passes (fused_kernel.hpp)
-    int sweeps = 1;
-    int n_strips2 = 0, n_segs2 = 0, seg_rows2 = 0, n_waves2 = 0;
-    double *v = nullptr, *xc = nullptr, *yc = nullptr;
-    double *div = nullptr;            // two divisor slots (imaginary time)
-    double *partials = nullptr;       // one per wave
-    unsigned *counter = nullptr;
-    double *obs_partials = nullptr;
-    int obs_blocks = 0;
-    int cur = 0, div_slot = 0;
-    bool div_pending = false, has_state = false, has_e = false, has_v = false;
-    hipStream_t own = nullptr, stream = nullptr;
-    double aH = 0, bH = 0, aV = 0, bV = 0, c_nl = 0, c_lhy = 0;   // FP64, as the reference
-    // scaled-frame factors actually used by the kernel (values of the kernel precision)
-    double mH = 1, tH = 0, mV = 1, tV = 0, unH = 1, unV = 1;
-    double corr_re = 1.0, corr_im = 0.0;   // folded into E
-    double c_nl_s = 0, c_lhy_s = 0;        // couplings rescaled to the mid-step frame
-    int mode = 0;                          // 0 / 1: both axes in form A / B; -1: mixed
-    int ix0 = 0, ix1 = 0, iy0 = 0, iy1 = 0;
-    // reverse sweep (real time, closed axes, no nonlinearity)
-    bool adj_ok = false;
-    double amH = 1, atH = 0, amV = 1, atV = 0, aunH = 1, aunV = 1;   // frame of the step at -dt
-    double corr_abs = 1.0;                                         // |corr| = |kappa|^2
-    void *ck = nullptr;               // checkpoints: 2 planes each
-    size_t ck_cap = 0;
-    int ck_every = 0, ck_steps = 0, ck_count = 0;
-    bool ck_valid = false;
-    void *adj = nullptr;              // 8 planes: adjoint re/im x2, reversed state re/im x2
-    double *grad = nullptr;           // FP64, pitched
-};
 
 namespace {
 
@@ -621,6 +572,8 @@ int observables(tsmi_ctx *c, bool energy_margin, double out[OBS_COUNT]) {
 }
 
 }  // namespace
+
+int tsmi::ctx_exchange(tsmi_ctx *c, void *re, void *im) { return exchange_planes(c, re, im); }
 
 extern "C" {
 

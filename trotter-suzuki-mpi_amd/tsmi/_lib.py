@@ -24,6 +24,9 @@ EXPORTS = (
     "tsmi_batch_set_exp_potential", "tsmi_batch_evolve", "tsmi_batch_get_state", "tsmi_batch_tune",
     "tsmi_batch_info", "tsmi_batch_synchronize", "tsmi_kernel_timing", "tsmi_kernel_time", "tsmi_squared_norm", "tsmi_energy", "tsmi_expected_values",
     "tsmi_get_sample", "tsmi_get_state_device", "tsmi_synchronize", "tsmi_geometry", "tsmi_last_error",
+    "tsmi_pair_create", "tsmi_pair_destroy", "tsmi_pair_set_state", "tsmi_pair_set_exp_potential",
+    "tsmi_pair_set_potential", "tsmi_pair_evolve", "tsmi_pair_get_state", "tsmi_pair_energy",
+    "tsmi_pair_geometry", "tsmi_pair_synchronize",
 )
 
 
@@ -38,6 +41,11 @@ class Desc(C.Structure):
                 ("delta_x", C.c_double), ("delta_y", C.c_double),
                 ("length_x", C.c_double), ("length_y", C.c_double),
                 ("norm_target", C.c_double)]
+
+
+class PairDesc(C.Structure):
+    _fields_ = [("a", Desc), ("b", Desc), ("coupling_ab", C.c_double), ("omega_r", C.c_double),
+                ("omega_i", C.c_double)]
 
 
 class TsmiError(RuntimeError):
@@ -100,10 +108,21 @@ def load():
     L.tsmi_get_sample.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _dp, _dp]
     L.tsmi_synchronize.argtypes = [C.c_void_p]
     L.tsmi_geometry.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    L.tsmi_pair_create.argtypes = [C.POINTER(PairDesc), C.POINTER(C.c_void_p)]
+    L.tsmi_pair_destroy.argtypes = [C.c_void_p]
+    L.tsmi_pair_destroy.restype = None
+    L.tsmi_pair_set_state.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
+    L.tsmi_pair_set_exp_potential.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
+    L.tsmi_pair_set_potential.argtypes = [C.c_void_p, C.c_int32, _dp]
+    L.tsmi_pair_evolve.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    L.tsmi_pair_get_state.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
+    L.tsmi_pair_energy.argtypes = [C.c_void_p, _dp]
+    L.tsmi_pair_geometry.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    L.tsmi_pair_synchronize.argtypes = [C.c_void_p]
     L.tsmi_last_error.argtypes = []
     L.tsmi_last_error.restype = C.c_char_p
     for name in EXPORTS:
-        if name not in ("tsmi_destroy", "tsmi_batch_destroy", "tsmi_last_error"):
+        if name not in ("tsmi_destroy", "tsmi_batch_destroy", "tsmi_pair_destroy", "tsmi_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
