@@ -47,8 +47,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=12, help="timed steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-studies", action="store_true",
-                    help="skip the precision study, the batch and the adjoint measurements")
-    ap.add_argument("--only-study", choices=["precision_study", "batch", "adjoint"],
+                    help="skip the precision study, the batch, adjoint and two-component measurements")
+    ap.add_argument("--only-study", choices=["precision_study", "batch", "adjoint", "pair"],
                     help="only run that study and print its JSON object (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (for counter calibration) and exit")
@@ -230,6 +230,44 @@ def adjoint_study():
     return out
 
 
+def pair_study(n: int = 4096, steps: int = 200):
+    """Two-component real-time GPE with Rabi coupling on an n x n periodic
+    lattice (two seeded Thomas-Fermi clouds, g_a = g_b = 50, g_ab = 40,
+    omega = 1), FP32 and FP64: one pair_kernel pass per step (both components,
+    the Rabi rotation applied at the pass exit) plus the halo exchange."""
+    import numpy as np
+    from tsmi import workloads as W
+    from tsmi.lattice import Lattice2D
+    from tsmi.pair import HipPair
+    lat = Lattice2D(n, 64.0, periodic_x=True, periodic_y=True)
+    X, Y = lat.mesh()
+    v_a = 0.5 * 0.01 * (X * X + Y * Y)
+    v_b = 0.5 * 0.012 * (X * X + Y * Y)
+    amp = np.sqrt(np.maximum(10.0 - v_a, 0.0) / 50.0)
+    psi_a = (amp * np.exp(1j * 0.3 * X)).astype(np.complex128)
+    psi_b = (0.5 * amp * np.exp(-1j * 0.2 * Y)).astype(np.complex128)
+    pw = W.PairWorkload(f"pair_{n}", lat, psi_a, psi_b, v_a, v_b, 5e-4, (steps,), g_a=50.0, g_b=50.0,
+                        g_ab=40.0, omega_r=1.0)
+    sites = n * n
+    out = {"workload": f"two-component real-time GPE with Rabi coupling, {n}x{n} periodic, g_a = g_b = 50, "
+                       f"g_ab = 40, omega = 1, dt = 5e-4, {steps} steps", "inner_sites": sites}
+    for prec in (32, 64):
+        with HipPair(pw, precision=prec) as h:
+            h.evolve(4)
+            h.synchronize()
+            t = time.perf_counter()
+            h.evolve(steps)
+            h.synchronize()
+            sec = time.perf_counter() - t
+            e = h.energy()
+            geo = h.geometry()
+        alg = 2 * BYTES_PER_SU[prec] * sites * steps
+        out[f"fp{prec}"] = {"ms_per_step": sec * 1e3 / steps,
+                            "component_site_updates_per_s": 2 * sites * steps / sec,
+                            "algorithmic_GB_per_s": alg / sec / 1e9, "norm": e[11], "geometry": geo}
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -245,7 +283,8 @@ def main():
         calibrate_copy()
         return
     if args.only_study:
-        fn = {"precision_study": precision_study, "batch": batch_study, "adjoint": adjoint_study}[args.only_study]
+        fn = {"precision_study": precision_study, "batch": batch_study, "adjoint": adjoint_study,
+              "pair": pair_study}[args.only_study]
         print(json.dumps({args.only_study: fn()}), flush=True)
         return
     from tsmi import workloads_torch as WT
@@ -338,7 +377,8 @@ def main():
         line["fp32_rel_l2_vs_fp64"] = rel
         line["norm_drift"] = {"fp32": abs(results[32]["norm"] - 1.0), "fp64": abs(r64["norm"] - 1.0)}
     if not args.no_studies:
-        for name, fn in (("precision_study", precision_study), ("batch", batch_study), ("adjoint", adjoint_study)):
+        for name, fn in (("precision_study", precision_study), ("batch", batch_study), ("adjoint", adjoint_study),
+                         ("pair", pair_study)):
             log(f"{name} ...")
             t = time.perf_counter()
             line[name] = fn()

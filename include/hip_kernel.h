@@ -12,6 +12,16 @@
  *
  * Supported: one component, Cartesian coordinates, angular velocity 0, real or
  * imaginary time, closed or periodic axes, FP64 or FP32 bulk.
+ *
+ * HipPairKernel is the sibling of the two-state CPUBlock constructor
+ * (src/cpukernel.cpp:274-392) for a two-component system, with the same
+ * arguments.  The Solver drives a step as run_kernel + wait_for_completion for
+ * each component, then rabi_coupling(var) and normalization()
+ * (src/solver.cpp:201-226); the device runs both components' steps as one pass,
+ * so HipPairKernel queues the two run_kernel calls and issues the pass when
+ * rabi_coupling supplies var (real time: the rotation is applied as the rows
+ * leave the pass; imaginary time: the rotation and the joint renormalisation
+ * follow as one more pass at normalization()).
  */
 #ifndef TSMI_HIP_KERNEL_H
 #define TSMI_HIP_KERNEL_H
@@ -56,6 +66,47 @@ private:
     int precision_;
     int width_, height_;
     bool exchange_next_;
+};
+
+class HipPairKernel : public ITrotterKernel {
+public:
+    HipPairKernel(Lattice *grid, State *state1, State *state2, Hamiltonian2Component *hamiltonian,
+                  double **external_pot_real, double **external_pot_imag, double delta_t, double *norm,
+                  bool imag_time, int precision = 64);
+    ~HipPairKernel();
+
+    void run_kernel();                 ///< queues the current component's step
+    void run_kernel_on_halo() {}       ///< the step covers the whole tile
+    void wait_for_completion();        ///< moves to the other component
+    void get_sample(size_t dest_stride, size_t x, size_t y, size_t width, size_t height,
+                    double *dest_real, double *dest_imag, double *dest_real2 = 0, double *dest_imag2 = 0) const;
+    void normalization();
+    void rabi_coupling(double var, double delta_t);
+    double calculate_squared_norm(bool global = true) const;   ///< of the current component
+    bool runs_in_place() const { return false; }
+    std::string get_name() const { return precision_ == 32 ? "HIP-FP32" : "HIP"; }
+    void update_potential(double *_external_pot_real, double *_external_pot_imag, int which);
+    void cpy_first_positive_to_first_negative() {}
+    void start_halo_exchange();        ///< the queued steps are followed by the exchange
+    void finish_halo_exchange() {}
+
+    // Beyond the interface: used by this project's Solver.
+    void set_potential(int which, const double *v);   ///< raw V of one component, for energy()
+    void energy(double out[12]) const; ///< the two-component energies (tsmi_pair_energy)
+    tsmi_pair *pair() const { return pair_; }
+
+private:
+    void flush() const;                ///< issues queued work that no rabi_coupling followed
+    tsmi_pair *pair_;
+    int precision_;
+    int width_, height_;
+    bool imag_;
+    double delta_t_;
+    int state_index_;
+    mutable int queued_;
+    mutable bool exchange_;
+    mutable bool rot_pending_;
+    mutable double rot_var_;
 };
 
 #endif

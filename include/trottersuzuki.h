@@ -8,10 +8,13 @@
  * evolution runs on the GPU through libtsmi_hip.so (include/tsmi_hip.h); this
  * layer only builds the lattice, the state and the potential factor on the host.
  *
+ * Two-component systems (Hamiltonian2Component with the two-state Solver) run
+ * through HipPairKernel.
+ *
  * Outside this path (and rejected with std::runtime_error, as the reference's
  * my_abort does in a build without MPI): Lattice1D, cylindrical coordinates,
- * a rotating frame (angular_velocity != 0), two-component systems, kernel types
- * other than "hip"/"hip-fp32".
+ * a rotating frame (angular_velocity != 0), kernel types other than
+ * "hip"/"hip-fp32".
  */
 #ifndef TSMI_TROTTERSUZUKI_H
 #define TSMI_TROTTERSUZUKI_H
@@ -191,6 +194,22 @@ protected:
     Lattice *grid;
 };
 
+/** Two-component Hamiltonian (src/trottersuzuki.h:459-497, src/model.cpp:979-997). */
+class Hamiltonian2Component : public Hamiltonian {
+public:
+    double mass_b;
+    double coupling_ab;
+    double coupling_b;
+    double omega_r;
+    double omega_i;
+    Potential *potential_b;
+    Hamiltonian2Component(Lattice *grid, Potential *potential = 0, Potential *potential_b = 0, double mass = 1.,
+                          double mass_b = 1., double coupling_a = 0., double coupling_ab = 0.,
+                          double coupling_b = 0., double omega_r = 0, double omega_i = 0,
+                          double angular_velocity = 0., double rot_coord_x = 0, double rot_coord_y = 0);
+    ~Hamiltonian2Component();
+};
+
 /** Kernel interface (src/trottersuzuki.h:502-520). */
 class ITrotterKernel {
 public:
@@ -222,6 +241,8 @@ public:
     double current_evolution_time;
 
     Solver(Lattice *grid, State *state, Hamiltonian *hamiltonian, double delta_t, string kernel_type = "hip");
+    Solver(Lattice *grid, State *state1, State *state2, Hamiltonian2Component *hamiltonian, double delta_t,
+           string kernel_type = "hip");
     ~Solver();
     void evolve(int iterations, bool imag_time = false);
     void update_parameters();
@@ -232,19 +253,23 @@ public:
     double get_rotational_energy(size_t which = 3);
     double get_intra_species_energy(size_t which = 3);
     double get_LeeHuangYang_energy(void);
+    double get_inter_species_energy(void);
+    double get_rabi_energy(void);
     void set_exp_potential(double *real, int real_length, double *imag, int imag_length, int which);
 
 private:
     bool imag_time;
-    double *external_pot_real;
-    double *external_pot_imag;
+    double *external_pot_real[2];
+    double *external_pot_imag[2];
     double delta_t;
-    double norm2;
+    double norm2[2];
     string kernel_type;
     ITrotterKernel *kernel;
-    void initialize_exp_potential(double delta_t);
+    bool single_component;
+    void initialize_exp_potential(double delta_t, int which = 0);
     void init_kernel();
     double energies[6];     ///< total, kinetic, potential, intra-species, LHY, squared norm
+    double energies2[12];   ///< two components: as tsmi_pair_energy
     bool has_parameters_changed;
     bool energy_expected_values_updated;
     void calculate_energy_expected_values(void);

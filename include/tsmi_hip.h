@@ -213,6 +213,17 @@ int tsmi_pair_set_potential(tsmi_pair *pair, int32_t which, const double *v);
  * kernel the Solver keeps.  Asynchronous. */
 int tsmi_pair_evolve(tsmi_pair *pair, int32_t n_steps, int32_t initial_half);
 
+/* The same loop one piece at a time, for a kernel object driven by the
+ * reference's Solver through ITrotterKernel (src/trottersuzuki.h:502-520).
+ * tsmi_pair_step: both components' steps (run_kernel_on_halo + run_kernel +
+ * wait_for_completion of each, src/cpukernel.cpp:413-570), the halo exchange of
+ * both when exchange != 0, and in real time the following rabi_coupling(rabi_var)
+ * applied as the rows leave the step (rabi_var = 0: none).  tsmi_pair_rabi:
+ * rabi_coupling(var) of both current tiles (src/cpukernel.cpp:581-608) and, when
+ * normalize != 0 in imaginary time, normalization() (:610-667).  Asynchronous. */
+int tsmi_pair_step(tsmi_pair *pair, int32_t exchange, double rabi_var);
+int tsmi_pair_rabi(tsmi_pair *pair, double var, int32_t normalize);
+
 /* get_sample of one component, the whole tile (src/cpukernel.cpp:572-579); syncs. */
 int tsmi_pair_get_state(tsmi_pair *pair, int32_t which, double *re, double *im);
 

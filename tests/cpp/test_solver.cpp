@@ -1,10 +1,12 @@
-// test_solver.cpp -- the single-component scenarios of the reference's kernel
-// tests (test/kerneltest.cpp:8-125: DIM 250, LENGTH 100) on this project's
+// test_solver.cpp -- the scenarios of the reference's kernel tests
+// (test/kerneltest.cpp:8-125 single component, :178-237 two components; DIM 250,
+// LENGTH 100) on this project's
 // host API with a HIP kernel type, plus its error paths.  Prints one JSON line
 // per scenario; tests/test_cpp_api.py checks the values.
 //
 //   test_solver <kernel_type> [scenario]
 //   test_solver io case.bin prefix      (text I/O, no device work)
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -81,6 +83,33 @@ static void imaginary_intra_particle_interaction(const std::string &kt) {
     report("imaginary_intra_particle_interaction", kt, 0.0, solver.get_total_energy(), n0,
            solver.get_squared_norm(), state.get_mean_xx());
 }
+
+// The two-component scenarios (test/kerneltest.cpp:178-237) through the
+// two-state Solver: state 2 starts empty, the Rabi coupling moves the norm.
+static void report_pair(const char *name, const std::string &kt, double n0, double n, double n10, double n20,
+                        double n1, double n2) {
+    std::printf("{\"scenario\": \"%s\", \"kernel\": \"%s\", \"N0\": %.17g, \"N\": %.17g, \"N1_0\": %.17g, "
+                "\"N2_0\": %.17g, \"N1\": %.17g, \"N2\": %.17g}\n",
+                name, kt.c_str(), n0, n, n10, n20, n1, n2);
+    std::fflush(stdout);
+}
+
+static void mixed_bec(const std::string &kt, bool imag) {
+    Lattice2D grid(DIM, LENGTH);
+    GaussianState state1(&grid, 1);
+    State state2(&grid);
+    HarmonicPotential pot(&grid, 1., 1.);
+    Hamiltonian2Component ham(&grid, &pot, &pot, 1., 1., 0., 0., 0., 2. * M_PI / 10.);
+    Solver solver(&grid, &state1, &state2, &ham, 1.e-3, kt);
+    const double n0 = solver.get_squared_norm();
+    const double n10 = state1.get_squared_norm(), n20 = state2.get_squared_norm();
+    solver.evolve(imag ? 1000 : 5000, imag);
+    report_pair(imag ? "imaginary_mixed_BEC" : "mixed_BEC", kt, n0, solver.get_squared_norm(), n10, n20,
+                state1.get_squared_norm(), state2.get_squared_norm());
+}
+
+static void mixed_bec_real(const std::string &kt) { mixed_bec(kt, false); }
+static void mixed_bec_imag(const std::string &kt) { mixed_bec(kt, true); }
 
 // The error paths a caller can reach without a GPU fault (src/common.cpp:104-115:
 // std::runtime_error with a message).
@@ -160,6 +189,8 @@ int main(int argc, char **argv) {
         {"imaginary_harmonic_oscillator", imaginary_harmonic_oscillator},
         {"intra_particle_interaction", intra_particle_interaction},
         {"imaginary_intra_particle_interaction", imaginary_intra_particle_interaction},
+        {"mixed_BEC", mixed_bec_real},
+        {"imaginary_mixed_BEC", mixed_bec_imag},
     };
     try {
         if (only == "errors") {

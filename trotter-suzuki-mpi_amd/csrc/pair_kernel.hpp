@@ -34,7 +34,10 @@ struct PairParams {
     T mH[2], tH[2], mV[2], tV[2], unH[2], unV[2];   // scaled-frame pair factors per component
     T c_self[2];                     // dt g of each component, in its mid-step frame
     T c_ab;                          // dt g_ab (the other density is a stored, true-frame value)
-    T cc, cs_r, cs_i;                // real-time Rabi rotation of the rows leaving the pipelines
+    // Real-time Rabi rotation of the rows leaving the pipelines, in FP64 at every
+    // precision: R = [[c, i s], [i conj(s), c]] has R^H R = (c^2 + |s|^2) I, so
+    // constants rounded to FP32 would scale the norm by the same factor every step.
+    double cc, cs_r, cs_i;
     int rabi;                        // 1: apply it
     // imaginary time
     const double *div_in[2];         // pending divisors of the inputs (nullptr: 1)
@@ -224,14 +227,14 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
     auto emit = [&](int y, Row<T, VEC> &oa, Row<T, VEC> &ob) {
         if (!IMAG && p.rabi) {
             // src/cpucartesian.cpp:264-276, every right-hand side the old value
-            const T cc = p.cc, sr = p.cs_r, si = p.cs_i;
+            const double cc = p.cc, sr = p.cs_r, si = p.cs_i;
 #pragma unroll
             for (int k = 0; k < VEC; ++k) {
-                const T ar = oa.re[k], ai = oa.im[k], br = ob.re[k], bi = ob.im[k];
-                oa.re[k] = cc * ar - si * br - sr * bi;
-                oa.im[k] = cc * ai + sr * br - si * bi;
-                ob.re[k] = cc * br + si * ar - sr * ai;
-                ob.im[k] = cc * bi + sr * ar + si * ai;
+                const double ar = oa.re[k], ai = oa.im[k], br = ob.re[k], bi = ob.im[k];
+                oa.re[k] = T(cc * ar - si * br - sr * bi);
+                oa.im[k] = T(cc * ai + sr * br - si * bi);
+                ob.re[k] = T(cc * br + si * ar - sr * ai);
+                ob.im[k] = T(cc * bi + sr * ar + si * ai);
             }
         }
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
@@ -344,7 +347,7 @@ template <typename T>
 struct RabiParams {
     T *re[2], *im[2];            // both components' tiles, updated in place
     int W, H, pitch;
-    T cc, cs_r, cs_i;
+    double cc, cs_r, cs_i;       // FP64 at every precision (see PairParams)
     int rotate;                  // 0: only divide by the pending divisors
     const double *div_in[2];     // pending divisors (nullptr: 1)
     int normalize;               // the joint renormalisation of src/cpukernel.cpp:610-667
@@ -367,7 +370,7 @@ template <typename T, bool IMAG>
 __global__ void __launch_bounds__(kRabiThreads) rabi_kernel(RabiParams<T> p) {
     const double d0 = p.div_in[0] ? *p.div_in[0] : 1.0, d1 = p.div_in[1] ? *p.div_in[1] : 1.0;
     const bool div0_on = p.div_in[0] != nullptr, div1_on = p.div_in[1] != nullptr;
-    const T cc = p.cc, sr = p.cs_r, si = p.cs_i;
+    const double cc = p.cc, sr = p.cs_r, si = p.cs_i;
     double s0 = 0.0, c0 = 0.0, s1 = 0.0, c1 = 0.0;
     const long n = (long)p.W * (long)p.H;
     for (long i = (long)blockIdx.x * kRabiThreads + threadIdx.x; i < n; i += (long)gridDim.x * kRabiThreads) {
@@ -378,16 +381,17 @@ __global__ void __launch_bounds__(kRabiThreads) rabi_kernel(RabiParams<T> p) {
         if (div1_on) { br = T(double(br) / d1); bi = T(double(bi) / d1); }
         T nar = ar, nai = ai, nbr = br, nbi = bi;
         if (p.rotate) {
+            const double xr = ar, xi = ai, yr = br, yi = bi;
             if (IMAG) {   // src/cpucartesian.cpp:278-290
-                nar = cc * ar + sr * br - si * bi;
-                nai = cc * ai + si * br + sr * bi;
-                nbr = cc * br + sr * ar + si * ai;
-                nbi = cc * bi - si * ar + sr * ai;
+                nar = T(cc * xr + sr * yr - si * yi);
+                nai = T(cc * xi + si * yr + sr * yi);
+                nbr = T(cc * yr + sr * xr + si * xi);
+                nbi = T(cc * yi - si * xr + sr * xi);
             } else {      // src/cpucartesian.cpp:264-276
-                nar = cc * ar - si * br - sr * bi;
-                nai = cc * ai + sr * br - si * bi;
-                nbr = cc * br + si * ar - sr * ai;
-                nbi = cc * bi + sr * ar + si * ai;
+                nar = T(cc * xr - si * yr - sr * yi);
+                nai = T(cc * xi + sr * yr - si * yi);
+                nbr = T(cc * yr + si * xr - sr * xi);
+                nbi = T(cc * yi + sr * xr + si * xi);
             }
         }
         p.re[0][k] = nar; p.im[0][k] = nai;

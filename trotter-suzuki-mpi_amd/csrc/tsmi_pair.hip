@@ -93,8 +93,10 @@ bool rabi_on(const tsmi_pair *p) { return p->omega_r != 0.0 || p->omega_i != 0.0
 double *div_pair(tsmi_pair *p, int s) { return p->dv + 2 * s; }
 double *norms(tsmi_pair *p) { return p->dv + 4; }
 
+// One step of both components; in real time the Rabi rotation by var leaves the
+// pipelines with the rows when rotate.
 template <typename T, bool IMAG>
-int launch_pair_step(tsmi_pair *p, double var) {
+int launch_pair_step(tsmi_pair *p, double var, bool rotate) {
     PairParams<T> q;
     std::memset(&q, 0, sizeof(q));
     const int cur = p->c[0]->cur, nb = 1 - cur;
@@ -113,10 +115,10 @@ int launch_pair_step(tsmi_pair *p, double var) {
     q.W = p->W; q.H = p->H; q.pitch = p->pitch;
     q.n_strips = p->n_strips; q.n_waves = p->n_waves; q.seg_rows = p->seg_rows;
     q.c_ab = (T)p->c_ab;
-    if (!IMAG && rabi_on(p)) {
+    if (!IMAG && rotate) {
         double cc, sr, si;
         rabi_coefs(p, var, &cc, &sr, &si);
-        q.cc = (T)cc; q.cs_r = (T)sr; q.cs_i = (T)si;
+        q.cc = cc; q.cs_r = sr; q.cs_i = si;
         q.rabi = 1;
     }
     if (IMAG) {
@@ -159,7 +161,7 @@ int launch_rabi(tsmi_pair *p, double var, bool rotate, bool normalize) {
     if (rotate) {
         double cc, sr, si;
         rabi_coefs(p, var, &cc, &sr, &si);
-        q.cc = (T)cc; q.cs_r = (T)sr; q.cs_i = (T)si;
+        q.cc = cc; q.cs_r = sr; q.cs_i = si;
         q.rotate = 1;
     }
     if (p->pending) {
@@ -195,9 +197,28 @@ int rabi_pass(tsmi_pair *p, double var, bool rotate, bool normalize) {
     return p->esz == 4 ? launch_rabi<float>(p, var, rotate, normalize) : launch_rabi<double>(p, var, rotate, normalize);
 }
 
-int pair_step(tsmi_pair *p, double var) {
-    if (p->esz == 4) return p->imag ? launch_pair_step<float, true>(p, var) : launch_pair_step<float, false>(p, var);
-    return p->imag ? launch_pair_step<double, true>(p, var) : launch_pair_step<double, false>(p, var);
+int pair_step(tsmi_pair *p, double var, bool rotate) {
+    if (p->esz == 4)
+        return p->imag ? launch_pair_step<float, true>(p, var, rotate) : launch_pair_step<float, false>(p, var, rotate);
+    return p->imag ? launch_pair_step<double, true>(p, var, rotate) : launch_pair_step<double, false>(p, var, rotate);
+}
+
+// the periodic exchange of both components' current tiles
+int exchange_both(tsmi_pair *p) {
+    for (int c = 0; c < 2; ++c)
+        if (int st = ctx_exchange(p->c[c], p->c[c]->psi_re[p->c[c]->cur], p->c[c]->psi_im[p->c[c]->cur])) return st;
+    return TSMI_OK;
+}
+
+// Divides both current tiles by their pending divisors, if any.
+int settle(tsmi_pair *p) { return p->pending ? rabi_pass(p, 0.0, false, false) : TSMI_OK; }
+
+int check_ready(tsmi_pair *p, const char *who) {
+    if (!p) return set_error(TSMI_E_INVALID, std::string(who) + ": null pair");
+    for (int i = 0; i < 2; ++i)
+        if (!p->c[i]->has_state || !p->c[i]->has_e)
+            return set_error(TSMI_E_INVALID, std::string(who) + ": both states and E tables must be set");
+    return TSMI_OK;
 }
 
 template <typename T, bool IMAG>
@@ -320,11 +341,8 @@ int tsmi_pair_set_potential(tsmi_pair *p, int32_t which, const double *v) {
 }
 
 int tsmi_pair_evolve(tsmi_pair *p, int32_t n_steps, int32_t initial_half) {
-    if (!p) return set_error(TSMI_E_INVALID, "tsmi_pair_evolve: null pair");
+    if (int st = check_ready(p, "tsmi_pair_evolve")) return st;
     if (n_steps < 0) return set_error(TSMI_E_INVALID, "tsmi_pair_evolve: negative step count");
-    for (int i = 0; i < 2; ++i)
-        if (!p->c[i]->has_state || !p->c[i]->has_e)
-            return set_error(TSMI_E_INVALID, "tsmi_pair_evolve: both states and E tables must be set");
     const bool rabi = rabi_on(p);
     int st = TSMI_OK;
     // src/solver.cpp:173-177
@@ -332,32 +350,46 @@ int tsmi_pair_evolve(tsmi_pair *p, int32_t n_steps, int32_t initial_half) {
     for (int i = 0; i < n_steps && !st; ++i) {
         const bool last = i == n_steps - 1;
         const double var = last ? 0.5 : 1.0;   // src/solver.cpp:222-225
-        st = pair_step(p, var);
-        if (!st && !last) {
-            // the periodic exchange of both components (src/solver.cpp:203-219);
-            // the Rabi rotation and the renormalisations act cell by cell and
-            // commute with it
-            for (int c = 0; c < 2 && !st; ++c)
-                st = ctx_exchange(p->c[c], p->c[c]->psi_re[p->c[c]->cur], p->c[c]->psi_im[p->c[c]->cur]);
-        }
+        st = pair_step(p, var, rabi);
+        // the periodic exchange of both components (src/solver.cpp:203-219);
+        // the Rabi rotation and the renormalisations act cell by cell and
+        // commute with it
+        if (!st && !last) st = exchange_both(p);
         if (!st && p->imag && rabi) st = rabi_pass(p, var, true, true);   // src/solver.cpp:225-226
     }
     // the state at the end of a call is the renormalised one (src/cpukernel.cpp:560-563, :650-664)
-    if (!st && p->pending) st = rabi_pass(p, 0.0, false, false);
+    if (!st) st = settle(p);
     return st;
+}
+
+int tsmi_pair_step(tsmi_pair *p, int32_t exchange, double rabi_var) {
+    if (int st = check_ready(p, "tsmi_pair_step")) return st;
+    int st = pair_step(p, rabi_var, !p->imag && rabi_on(p) && rabi_var != 0.0);
+    if (!st && exchange) st = exchange_both(p);
+    return st;
+}
+
+int tsmi_pair_rabi(tsmi_pair *p, double var, int32_t normalize) {
+    if (int st = check_ready(p, "tsmi_pair_rabi")) return st;
+    const bool rot = rabi_on(p) && var != 0.0;
+    const bool nrm = normalize && p->imag && rabi_on(p);
+    if (!rot && !nrm) return TSMI_OK;
+    return rabi_pass(p, var, rot, nrm);
 }
 
 int tsmi_pair_get_state(tsmi_pair *p, int32_t which, double *re, double *im) {
     tsmi_ctx *c = component(p, which, "tsmi_pair_get_state");
     if (!c) return TSMI_E_INVALID;
     if (!re || !im) return set_error(TSMI_E_INVALID, "tsmi_pair_get_state: null argument");
+    if (int st = settle(p)) return st;
     return tsmi_get_sample(c, (size_t)c->W, 0, 0, (size_t)c->W, (size_t)c->H, re, im);
 }
 
 int tsmi_pair_energy(tsmi_pair *p, double out[12]) {
     if (!p || !out) return set_error(TSMI_E_INVALID, "tsmi_pair_energy: null argument");
     double ea[6], eb[6];
-    int st = tsmi_energy(p->c[0], ea);
+    int st = settle(p);
+    if (!st) st = tsmi_energy(p->c[0], ea);
     if (!st) st = tsmi_energy(p->c[1], eb);
     if (st) return st;
     tsmi_ctx *a = p->c[0], *b = p->c[1];

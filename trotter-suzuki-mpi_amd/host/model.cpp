@@ -459,3 +459,14 @@ Hamiltonian::Hamiltonian(Lattice *g, Potential *pot, double m, double ca, double
 Hamiltonian::~Hamiltonian() {
     if (self_init) delete potential;
 }
+
+// src/model.cpp:979-997: no LHY term; the second potential defaults to the first
+Hamiltonian2Component::Hamiltonian2Component(Lattice *g, Potential *pot, Potential *pot_b, double m, double m_b,
+                                             double ca, double cab, double cb, double om_r, double om_i,
+                                             double omega, double rx, double ry)
+    : Hamiltonian(g, pot, m, ca, 0., omega, rx, ry), mass_b(m_b), coupling_ab(cab), coupling_b(cb), omega_r(om_r),
+      omega_i(om_i) {
+    potential_b = pot_b ? pot_b : potential;
+}
+
+Hamiltonian2Component::~Hamiltonian2Component() {}

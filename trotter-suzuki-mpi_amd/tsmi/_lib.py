@@ -25,7 +25,8 @@ EXPORTS = (
     "tsmi_batch_info", "tsmi_batch_synchronize", "tsmi_kernel_timing", "tsmi_kernel_time", "tsmi_squared_norm", "tsmi_energy", "tsmi_expected_values",
     "tsmi_get_sample", "tsmi_get_state_device", "tsmi_synchronize", "tsmi_geometry", "tsmi_last_error",
     "tsmi_pair_create", "tsmi_pair_destroy", "tsmi_pair_set_state", "tsmi_pair_set_exp_potential",
-    "tsmi_pair_set_potential", "tsmi_pair_evolve", "tsmi_pair_get_state", "tsmi_pair_energy",
+    "tsmi_pair_set_potential", "tsmi_pair_evolve", "tsmi_pair_step", "tsmi_pair_rabi", "tsmi_pair_get_state",
+    "tsmi_pair_energy",
     "tsmi_pair_geometry", "tsmi_pair_synchronize",
 )
 
@@ -115,6 +116,8 @@ def load():
     L.tsmi_pair_set_exp_potential.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
     L.tsmi_pair_set_potential.argtypes = [C.c_void_p, C.c_int32, _dp]
     L.tsmi_pair_evolve.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    L.tsmi_pair_step.argtypes = [C.c_void_p, C.c_int32, C.c_double]
+    L.tsmi_pair_rabi.argtypes = [C.c_void_p, C.c_double, C.c_int32]
     L.tsmi_pair_get_state.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
     L.tsmi_pair_energy.argtypes = [C.c_void_p, _dp]
     L.tsmi_pair_geometry.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
