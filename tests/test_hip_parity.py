@@ -199,3 +199,52 @@ def test_two_steps_per_pass_rejected_where_not_exact():
     for wl in (W.small_case(32, 32, True, False, steps=2), W.small_case(32, 32, True, True, steps=2, imag_time=True)):
         with pytest.raises(_lib.TsmiError, match="two steps per pass"):
             kernel_for(wl, sweeps=2)
+
+
+def run_hip_with_e(wl, e, precision=64, sweeps=0, norm_target=0.0):
+    from tsmi.hip import kernel_for
+    k = kernel_for(wl, precision=precision, norm_target=norm_target, e_planes=e, sweeps=sweeps)
+    try:
+        k.evolve(wl.steps)
+        return k.sample()
+    finally:
+        k.close()
+
+
+@pytest.mark.parametrize("sweeps", [1, 2])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_non_unit_e_table(periodic, sweeps, oracle):
+    """An E table that is not a pure phase (a complex potential: |E| < 1, varying
+    across the tile) is applied as given: the steps then read both E planes
+    instead of the one-plane phase form, which would keep the norm."""
+    wl = W.small_case(96, 72, periodic, periodic, steps=9, g=2.0)
+    er, ei = oracle.exp_potential(wl.v, wl.dt, False)
+    damp = 0.999 - 0.01 * np.linspace(0.0, 1.0, er.shape[1])[None, :]
+    er, ei = er * damp, ei * damp
+    d = oracle.make_desc(wl)
+    re = np.ascontiguousarray(wl.psi.real).copy()
+    im = np.ascontiguousarray(wl.psi.imag).copy()
+    import ctypes as C
+    oracle.lib().ts_oracle_evolve(C.byref(d), wl.steps, oracle._p(re), oracle._p(im), oracle._p(er), oracle._p(ei))
+    ref = re + 1j * im
+    psi = run_hip_with_e(wl, (er, ei), sweeps=sweeps)
+    assert rel_l2(psi, ref) < 1e-12
+    # the same table read as a pure phase would keep the norm: it must drop
+    lat = wl.lattice
+    assert W.squared_norm(lat, psi) < W.squared_norm(lat, wl.psi) * (1 - 1e-4)
+
+
+def test_imaginary_time_e_with_zero(oracle):
+    """Imaginary time with an E table that has zeros (a hard wall) has no
+    exponent form: the steps read E itself."""
+    wl = W.small_case(64, 64, False, False, steps=7, imag_time=True, g=3.0)
+    er, ei = oracle.exp_potential(wl.v, wl.dt, True)
+    er = er.copy()
+    er[20:24, 30:34] = 0.0
+    d = oracle.make_desc(wl)
+    re = np.ascontiguousarray(wl.psi.real).copy()
+    im = np.ascontiguousarray(wl.psi.imag).copy()
+    import ctypes as C
+    oracle.lib().ts_oracle_evolve(C.byref(d), wl.steps, oracle._p(re), oracle._p(im), oracle._p(er), oracle._p(ei))
+    psi = run_hip_with_e(wl, (er, ei), norm_target=d.norm_target)
+    assert rel_l2(psi, re + 1j * im) < 1e-12

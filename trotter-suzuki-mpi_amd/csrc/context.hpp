@@ -25,6 +25,11 @@ struct tsmi_ctx {
     void *psi_base[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     void *e_base[2] = {nullptr, nullptr};
     void *e8[2] = {nullptr, nullptr};     // E wrapped over the whole 8-wide halo (fused passes)
+    // E corr as one plane (phase in real time, exponent in imaginary time): the
+    // steps read it instead of E when ph_mode (every |E| = 1, or every E > 0)
+    void *ph_base = nullptr, *ph = nullptr, *ph8 = nullptr;
+    bool ph_mode = false;
+    unsigned *flag = nullptr;             // set by the E import when E has no phase form
     double *v_base = nullptr;
     int Wd = 0, Hd = 0;
     size_t off_el = 0;                    // elements from allocation to view origin

@@ -29,10 +29,10 @@ struct Pipe {
 // One beat of a real-time step pipeline (the generic form of step_wave's beat):
 // input row t (in), E row t-2 (e), row validity v[k] = valid(t-k); returns row
 // t-4 in `out`.
-template <int TP, bool LHY, typename T, int VEC>
+template <int TP, bool LHY, bool PH, typename T, int VEC>
 __device__ __forceinline__ void pipe_beat(Pipe<T, VEC> &q, Row<T, VEC> in, const Row<T, VEC> &e, Row<T, VEC> &out,
                                           bool v0, bool v1, bool v2, bool v3, bool v4, const HCoef<T, VEC> &hc,
-                                          T mV, T tV, T unV, T c, T cl) {
+                                          T mV, T tV, T unV, T c, T cl, double mag) {
     const T a1 = (v0 && v1) ? mV : unV, b1 = (v0 && v1) ? tV : T(0);
     const T a2 = (v1 && v2) ? mV : unV, b2 = (v1 && v2) ? tV : T(0);
     const T a3 = (v2 && v3) ? mV : unV, b3 = (v2 && v3) ? tV : T(0);
@@ -45,7 +45,7 @@ __device__ __forceinline__ void pipe_beat(Pipe<T, VEC> &q, Row<T, VEC> in, const
     Row<T, VEC> o2 = q.p2;
     q.p2 = o1;
     hpass<(TP + 1) & 1, false>(o2, hc);
-    ppass<false, LHY>(o2, e, c, cl);
+    ppass<false, LHY, PH>(o2, e, c, cl, mag);
     hpass<(TP + 1) & 1, false>(o2, hc);
     vpair<TP & 1, false>(q.p3, o2, a3, b3);
     Row<T, VEC> o3 = q.p3;
@@ -57,9 +57,9 @@ __device__ __forceinline__ void pipe_beat(Pipe<T, VEC> &q, Row<T, VEC> in, const
 }
 
 // The same beat when the strip is interior and all five rows are valid.
-template <int TP, bool LHY, int MODE, typename T, int VEC>
+template <int TP, bool LHY, int MODE, bool PH, typename T, int VEC>
 __device__ __forceinline__ void pipe_beat_fast(Pipe<T, VEC> &q, Row<T, VEC> in, const Row<T, VEC> &e,
-                                               Row<T, VEC> &out, T kH, T kV, T c, T cl) {
+                                               Row<T, VEC> &out, T kH, T kV, T c, T cl, double mag) {
     constexpr int M = MODE < 0 ? 0 : MODE;
     vpair_fast<(TP + 1) & 1, false, M>(q.p1, in, kV);
     Row<T, VEC> o1 = q.p1;
@@ -69,7 +69,7 @@ __device__ __forceinline__ void pipe_beat_fast(Pipe<T, VEC> &q, Row<T, VEC> in, 
     Row<T, VEC> o2 = q.p2;
     q.p2 = o1;
     hpass_fast<(TP + 1) & 1, false, M>(o2, kH);
-    ppass<false, LHY>(o2, e, c, cl);
+    ppass<false, LHY, PH>(o2, e, c, cl, mag);
     hpass_fast<(TP + 1) & 1, false, M>(o2, kH);
     vpair_fast<TP & 1, false, M>(q.p3, o2, kV);
     Row<T, VEC> o3 = q.p3;
@@ -83,7 +83,7 @@ __device__ __forceinline__ void pipe_beat_fast(Pipe<T, VEC> &q, Row<T, VEC> in, 
 // Two real-time steps of one wavefront's strip segment.  p.seg_rows, p.n_strips
 // and p.n_waves describe the fused decomposition (strips of kWave*VEC - 16
 // output columns).
-template <typename T, int VEC, bool LHY, int MODE>
+template <typename T, int VEC, bool LHY, int MODE, bool PH = false>
 __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wave, const int lane) {
     constexpr int AP = 2 * kApron;
     constexpr int SW = kWave * VEC - 2 * AP;
@@ -108,8 +108,8 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
     const unsigned seg_bytes = (unsigned)((long)(y_hi - y_lo) * p.pitch * (long)sizeof(T));
     const __amdgpu_buffer_rsrc_t r_in_re = make_rsrc(p.in_re + seg0, seg_bytes);
     const __amdgpu_buffer_rsrc_t r_in_im = make_rsrc(p.in_im + seg0, seg_bytes);
-    const __amdgpu_buffer_rsrc_t r_e_re = make_rsrc(p.e_re + seg0, seg_bytes);
-    const __amdgpu_buffer_rsrc_t r_e_im = make_rsrc(p.e_im + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_e_re = make_rsrc(PH ? p.ph + seg0 : p.e_re + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_e_im = make_rsrc(PH ? p.ph + seg0 : p.e_im + seg0, seg_bytes);
     const __amdgpu_buffer_rsrc_t r_out_re = make_rsrc(p.out_re + seg0, seg_bytes);
     const __amdgpu_buffer_rsrc_t r_out_im = make_rsrc(p.out_im + seg0, seg_bytes);
     const unsigned row_bytes = (unsigned)p.pitch * (unsigned)sizeof(T);
@@ -121,7 +121,14 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
         buf_load<T, VEC>(rr, v_ld, so, r.re);
         buf_load<T, VEC>(ri, v_ld, so, r.im);
     };
+    // E row (both planes), or its phase row (one plane) in PH form
+    auto load_e = [&](int y, Row<T, VEC> &r) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_load<T, VEC>(r_e_re, v_ld, so, r.re);
+        if (!PH) buf_load<T, VEC>(r_e_im, v_ld, so, r.im);
+    };
     auto valid = [&](int y) { return y >= y_lo && y < y_hi; };
+    const double mag = p.mag;
 
     Pipe<T, VEC> A, B;
     Row<T, VEC> inA, inB, eA, eB, fA, fB;
@@ -146,18 +153,18 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
     auto beat = [&](auto tp, int t, const Row<T, VEC> &in, const Row<T, VEC> &ea, const Row<T, VEC> &eb) {
         constexpr int TP = decltype(tp)::value;
         Row<T, VEC> a, b;
-        pipe_beat<TP, LHY>(A, in, ea, a, valid(t), valid(t - 1), valid(t - 2), valid(t - 3), valid(t - 4), hc, mV, tV,
-                           unV, c, cl);
-        pipe_beat<TP, LHY>(B, a, eb, b, valid(t - 4), valid(t - 5), valid(t - 6), valid(t - 7), valid(t - 8), hc, mV,
-                           tV, unV, c, cl);
+        pipe_beat<TP, LHY, PH>(A, in, ea, a, valid(t), valid(t - 1), valid(t - 2), valid(t - 3), valid(t - 4), hc, mV,
+                               tV, unV, c, cl, mag);
+        pipe_beat<TP, LHY, PH>(B, a, eb, b, valid(t - 4), valid(t - 5), valid(t - 6), valid(t - 7), valid(t - 8), hc,
+                               mV, tV, unV, c, cl, mag);
         const int y = t - 8;
         if (valid(y) && y >= y_out_lo && y < y_out_hi) emit(y, b);
     };
     auto beat_fast = [&](auto tp, int t, const Row<T, VEC> &in, const Row<T, VEC> &ea, const Row<T, VEC> &eb) {
         constexpr int TP = decltype(tp)::value;
         Row<T, VEC> a, b;
-        pipe_beat_fast<TP, LHY, MODE>(A, in, ea, a, kH, kV, c, cl);
-        pipe_beat_fast<TP, LHY, MODE>(B, a, eb, b, kH, kV, c, cl);
+        pipe_beat_fast<TP, LHY, MODE, PH>(A, in, ea, a, kH, kV, c, cl, mag);
+        pipe_beat_fast<TP, LHY, MODE, PH>(B, a, eb, b, kH, kV, c, cl, mag);
         const int y = t - 8;
         if (y >= y_out_lo && y < y_out_hi) emit(y, b);
     };
@@ -172,31 +179,31 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
         if (interior && t >= y_lo + 8 && t + 1 < y_hi) {
             beat_fast(I0(), t, inA, eA, fA);
             if (t + 2 < y_hi) load_row(r_in_re, r_in_im, t + 2, inA);
-            load_row(r_e_re, r_e_im, t, eA);
-            load_row(r_e_re, r_e_im, t - 4, fA);
+            load_e(t, eA);
+            load_e(t - 4, fA);
             beat_fast(I1(), t + 1, inB, eB, fB);
             if (t + 3 < y_hi) load_row(r_in_re, r_in_im, t + 3, inB);
-            load_row(r_e_re, r_e_im, t + 1, eB);
-            load_row(r_e_re, r_e_im, t - 3, fB);
+            load_e(t + 1, eB);
+            load_e(t - 3, fB);
         } else {
             beat(I0(), t, inA, eA, fA);
             if (t + 2 < y_hi) load_row(r_in_re, r_in_im, t + 2, inA);
-            if (t < y_hi) load_row(r_e_re, r_e_im, t, eA);
-            if (valid(t - 4)) load_row(r_e_re, r_e_im, t - 4, fA);
+            if (t < y_hi) load_e(t, eA);
+            if (valid(t - 4)) load_e(t - 4, fA);
             beat(I1(), t + 1, inB, eB, fB);
             if (t + 3 < y_hi) load_row(r_in_re, r_in_im, t + 3, inB);
-            if (t + 1 < y_hi) load_row(r_e_re, r_e_im, t + 1, eB);
-            if (valid(t - 3)) load_row(r_e_re, r_e_im, t - 3, fB);
+            if (t + 1 < y_hi) load_e(t + 1, eB);
+            if (valid(t - 3)) load_e(t - 3, fB);
         }
     }
 }
 
-template <typename T, int VEC, bool LHY, int MODE>
+template <typename T, int VEC, bool LHY, int MODE, bool PH = false>
 __global__ void __launch_bounds__(256) step2_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
-    step2_wave<T, VEC, LHY, MODE>(p, wave, lane);
+    step2_wave<T, VEC, LHY, MODE, PH>(p, wave, lane);
 }
 
 }  // namespace tsmi

@@ -46,6 +46,11 @@ struct StepParams {
     const T *in_re, *in_im;
     T *out_re, *out_im;
     const T *e_re, *e_im;        // E = exp(-i dt V) planes (e_im unused in imaginary time)
+    // The same factor as one plane, when every |E| = 1 (real time) or E > 0
+    // (imaginary time): phi = -arg(E corr), or lambda = -ln(E corr); the step
+    // then applies mag exp(-i (phase + phi)), or exp(-(decay + lambda)).
+    const T *ph;
+    double mag;                  // |corr| (real time, phase form)
     int W, H, pitch;             // tile and row pitch (elements)
     int n_strips, n_waves, seg_rows;
     // Pair factors in the scaled frame (see the note at fpair): a pair becomes
@@ -356,14 +361,19 @@ __device__ __forceinline__ void expneg_row(const T (&x)[VEC], T (&e)[VEC]) {
     }
 }
 
-// P on one row (src/cpucartesian.cpp:93-109 / :127-137).
-template <bool IMAG, bool LHY, typename T, int VEC>
-__device__ __forceinline__ void ppass(Row<T, VEC> &r, const Row<T, VEC> &e, T c, T cl) {
+// P on one row (src/cpucartesian.cpp:93-109 / :127-137).  PH: e.re holds the
+// phase phi (real time) or the exponent lambda (imaginary time) of E corr, which
+// joins the nonlinear angle: one sincos (or exp) gives both factors, and |E|
+// never differs from 1 by a rounding that repeats at the same site every step.
+// The modulus |corr| is applied in FP64 before the factors are rounded.
+template <bool IMAG, bool LHY, bool PH = false, typename T, int VEC>
+__device__ __forceinline__ void ppass(Row<T, VEC> &r, const Row<T, VEC> &e, T c, T cl, double mag = 1.0) {
     T ang[VEC];
 #pragma unroll
     for (int k = 0; k < VEC; ++k) {
         const T n2 = r.re[k] * r.re[k] + r.im[k] * r.im[k];
         ang[k] = LHY ? c * n2 + cl * (n2 * sqrt_t(n2)) : c * n2;
+        if (PH) ang[k] += e.re[k];
     }
     if (!IMAG) {
         T sn[VEC], cs[VEC];
@@ -371,18 +381,39 @@ __device__ __forceinline__ void ppass(Row<T, VEC> &r, const Row<T, VEC> &e, T c,
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
             const T pr = r.re[k], pi = r.im[k];
-            const T r1 = e.re[k] * pr - e.im[k] * pi;
-            const T i1 = e.re[k] * pi + e.im[k] * pr;
-            r.re[k] = cs[k] * r1 + sn[k] * i1;
-            r.im[k] = cs[k] * i1 - sn[k] * r1;
+            if (PH) {
+                T c2, s2;
+                if constexpr (sizeof(T) == 4) {
+                    // |corr| differs from 1 by less than half an FP32 ulp near 1, so
+                    // cos * |corr| rounded alone would drop it at almost every site
+                    // (a norm bias of ~6e-9 per step); renormalising (cos, sin) in
+                    // FP64 first (first order in cos^2 + sin^2 - 1) dithers that
+                    // rounding and leaves no bias above ~1e-10 per step
+                    const double cd = cs[k], sd = sn[k];
+                    const double kk = mag * (1.5 - 0.5 * (cd * cd + sd * sd));
+                    c2 = T(cd * kk);
+                    s2 = T(sd * kk);
+                } else {
+                    c2 = cs[k] * mag;
+                    s2 = sn[k] * mag;
+                }
+                r.re[k] = c2 * pr + s2 * pi;
+                r.im[k] = c2 * pi - s2 * pr;
+            } else {
+                const T r1 = e.re[k] * pr - e.im[k] * pi;
+                const T i1 = e.re[k] * pi + e.im[k] * pr;
+                r.re[k] = cs[k] * r1 + sn[k] * i1;
+                r.im[k] = cs[k] * i1 - sn[k] * r1;
+            }
         }
     } else {
         T f[VEC];
         expneg_row<T, VEC>(ang, f);
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
-            r.re[k] = f[k] * e.re[k] * r.re[k];
-            r.im[k] = f[k] * e.re[k] * r.im[k];
+            const T g = PH ? f[k] : f[k] * e.re[k];
+            r.re[k] = PH ? g * r.re[k] : f[k] * e.re[k] * r.re[k];
+            r.im[k] = PH ? g * r.im[k] : f[k] * e.re[k] * r.im[k];
         }
     }
 }
@@ -482,7 +513,7 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
 // ---------------------------------------------------------------------------
 // One wavefront's share of a step: strip (wave mod n_strips), segment
 // (wave / n_strips).  `wave` is wave-uniform.
-template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
+template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false>
 __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave, const int lane) {
     constexpr int SW = kWave * VEC - 2 * kApron;       // output columns per strip
     const int strip = wave % p.n_strips;
@@ -516,8 +547,9 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     const unsigned seg_bytes = (unsigned)((long)(y_hi - y_lo) * p.pitch * (long)sizeof(T));
     const __amdgpu_buffer_rsrc_t r_in_re = make_rsrc(p.in_re + seg0, seg_bytes);
     const __amdgpu_buffer_rsrc_t r_in_im = make_rsrc(p.in_im + seg0, seg_bytes);
-    const __amdgpu_buffer_rsrc_t r_e_re = make_rsrc(p.e_re + seg0, seg_bytes);
-    const __amdgpu_buffer_rsrc_t r_e_im = make_rsrc(IMAG ? p.e_re + seg0 : p.e_im + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t r_e_re = make_rsrc(PH ? p.ph + seg0 : p.e_re + seg0, seg_bytes);
+    // the second E plane exists only for complex E (real time, E form)
+    const __amdgpu_buffer_rsrc_t r_e_im = make_rsrc((IMAG || PH) ? (PH ? p.ph : p.e_re) + seg0 : p.e_im + seg0, seg_bytes);
     const __amdgpu_buffer_rsrc_t r_out_re = make_rsrc(p.out_re + seg0, seg_bytes);
     const __amdgpu_buffer_rsrc_t r_out_im = make_rsrc(p.out_im + seg0, seg_bytes);
     const unsigned row_bytes = (unsigned)p.pitch * (unsigned)sizeof(T);
@@ -546,7 +578,7 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     auto load_e = [&](int y, Row<T, VEC> &r) {
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
         buf_load<T, VEC>(r_e_re, v_ld, so, r.re);
-        if (!IMAG) buf_load<T, VEC>(r_e_im, v_ld, so, r.im);
+        if (!IMAG && !PH) buf_load<T, VEC>(r_e_im, v_ld, so, r.im);
     };
 
     // Every row register starts at zero: rows outside the segment stay finite,
@@ -601,7 +633,7 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         Row<T, VEC> o2 = pend2;
         pend2 = o1;
         hpass<(TP + 1) & 1, IMAG>(o2, hc);
-        ppass<IMAG, LHY>(o2, e, p.c_nl, p.c_lhy);
+        ppass<IMAG, LHY, PH>(o2, e, p.c_nl, p.c_lhy, p.mag);
         hpass<(TP + 1) & 1, IMAG>(o2, hc);
         // stage 3: V1, then H0 on row t-3
         vpair<TP & 1, IMAG>(pend3, o2, a3, b3);
@@ -628,7 +660,7 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         Row<T, VEC> o2 = pend2;
         pend2 = o1;
         hpass_fast<(TP + 1) & 1, IMAG, M>(o2, kH);
-        ppass<IMAG, LHY>(o2, e, p.c_nl, p.c_lhy);
+        ppass<IMAG, LHY, PH>(o2, e, p.c_nl, p.c_lhy, p.mag);
         hpass_fast<(TP + 1) & 1, IMAG, M>(o2, kH);
         vpair_fast<TP & 1, IMAG, M>(pend3, o2, kV);
         Row<T, VEC> o3 = pend3;
@@ -723,14 +755,14 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     }
 }
 
-template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
+template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false>
 __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform by construction; readfirstlane lets the compiler keep every
     // quantity derived from it (segment, strip, row bounds) in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
-    step_wave<T, IMAG, VEC, LHY, MODE>(p, wave, lane);
+    step_wave<T, IMAG, VEC, LHY, MODE, PH>(p, wave, lane);
 }
 
 // ---------------------------------------------------------------------------

@@ -120,8 +120,8 @@ int free_all(tsmi_ctx *c) {
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 2; ++k)
             if (c->psi_base[b][k]) (void)hipFree(c->psi_base[b][k]);
-    void *ptrs[] = {c->e_base[0], c->e_base[1], c->e8[0], c->e8[1], c->v_base, c->xc, c->yc, c->div, c->partials,
-                    c->counter, c->obs_partials, c->ck, c->adj, c->grad};
+    void *ptrs[] = {c->e_base[0], c->e_base[1], c->e8[0], c->e8[1], c->ph_base, c->ph8, c->flag, c->v_base, c->xc,
+                    c->yc, c->div, c->partials, c->counter, c->obs_partials, c->ck, c->adj, c->grad};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &k : c->ev)
@@ -184,26 +184,49 @@ __global__ void export_plane(const T *__restrict__ src, double *__restrict__ dst
 // E table of the step: E*corr computed in FP64, each component then rounded
 // once to the kernel precision.  corr is the step's product of the scaled-frame
 // factors (see tsmi_create).  sim may be null (imaginary time: E is real).
+// The same pass writes the one-plane form: phi = -arg(E corr) when |E| = 1
+// (real time), lambda = -ln(E corr) when E > 0 (imaginary time); a cell without
+// that form sets *bad.
 template <typename T>
 __global__ void import_exp(const double *__restrict__ sre, const double *__restrict__ sim, double cr, double ci,
-                           T *__restrict__ dre, T *__restrict__ dim, int W, int H, int pitch) {
+                           T *__restrict__ dre, T *__restrict__ dim, T *__restrict__ dph, unsigned *bad, int W,
+                           int H, int pitch) {
     const long n = (long)W * H;
+    bool ok = true;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const long y = i / W, x = i - y * W;
         const double er = sre[i], ei = sim ? sim[i] : 0.0;
-        dre[y * pitch + x] = (T)(er * cr - ei * ci);
-        dim[y * pitch + x] = (T)(er * ci + ei * cr);
+        const double zr = er * cr - ei * ci, zi = er * ci + ei * cr;
+        dre[y * pitch + x] = (T)zr;
+        dim[y * pitch + x] = (T)zi;
+        double ph = 0.0;
+        if (sim) {
+            ok = ok && fabs(er * er + ei * ei - 1.0) <= 1e-12;
+            ph = -atan2(zi, zr);
+        } else {
+            ok = ok && zr > 0.0;
+            ph = zr > 0.0 ? -log(zr) : 0.0;
+        }
+        dph[y * pitch + x] = (T)ph;
     }
+    if (!ok) atomicOr(bad, 1u);
 }
 
+// Imports E (FP64 W x H on the device) and decides whether the steps read its
+// one-plane form; syncs to read the import's flag.
 int launch_import_exp(tsmi_ctx *c, const double *re, const double *im) {
+    TSMI_HIP(hipMemsetAsync(c->flag, 0, sizeof(unsigned), c->stream));
     if (c->esz == 4)
         hipLaunchKernelGGL(import_exp<float>, dim3(2048), dim3(256), 0, c->stream, re, im, c->corr_re, c->corr_im,
-                           (float *)c->e_re, (float *)c->e_im, c->W, c->H, c->pitch);
+                           (float *)c->e_re, (float *)c->e_im, (float *)c->ph, c->flag, c->W, c->H, c->pitch);
     else
         hipLaunchKernelGGL(import_exp<double>, dim3(2048), dim3(256), 0, c->stream, re, im, c->corr_re, c->corr_im,
-                           (double *)c->e_re, (double *)c->e_im, c->W, c->H, c->pitch);
+                           (double *)c->e_re, (double *)c->e_im, (double *)c->ph, c->flag, c->W, c->H, c->pitch);
     TSMI_HIP(hipGetLastError());
+    unsigned bad = 1;
+    TSMI_HIP(hipMemcpyAsync(&bad, c->flag, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+    TSMI_HIP(hipStreamSynchronize(c->stream));
+    c->ph_mode = bad == 0;
     return TSMI_OK;
 }
 
@@ -213,9 +236,11 @@ int launch_import_exp(tsmi_ctx *c, const double *re, const double *im) {
 // goes to the pitched FP64 plane used by the energies.
 template <typename T>
 __global__ void exp_from_potential(const double *__restrict__ v, double dt, int imag, double cr, double ci,
-                                   T *__restrict__ dre, T *__restrict__ dim, double *__restrict__ vout, int W, int H,
-                                   int pitch) {
+                                   T *__restrict__ dre, T *__restrict__ dim, T *__restrict__ dph,
+                                   double *__restrict__ vout, int W, int H, int pitch) {
     const long n = (long)W * H;
+    // one-plane form: phi = dt V - arg(corr), lambda = dt V - ln(corr)
+    const double shift = imag ? log(cr) : atan2(ci, cr);
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const long y = i / W, x = i - y * W;
         const double vi = v[i];
@@ -228,6 +253,7 @@ __global__ void exp_from_potential(const double *__restrict__ v, double dt, int 
         }
         dre[y * pitch + x] = (T)(er * cr - ei * ci);
         dim[y * pitch + x] = (T)(er * ci + ei * cr);
+        dph[y * pitch + x] = (T)(dt * vi - shift);
         vout[y * pitch + x] = vi;
     }
 }
@@ -274,7 +300,7 @@ int exchange_halo(tsmi_ctx *c, void *base_re, void *base_im) {
 }
 
 // Two real-time steps in one pass over the whole allocation (fused_kernel.hpp).
-template <typename T, int VEC, bool LHY, int MODE>
+template <typename T, int VEC, bool LHY, int MODE, bool PH>
 int launch_step2(tsmi_ctx *c) {
     StepParams<T> p;
     std::memset(&p, 0, sizeof(p));
@@ -285,6 +311,8 @@ int launch_step2(tsmi_ctx *c) {
     p.out_im = (T *)c->psi_base[nb][1];
     p.e_re = (const T *)(c->halo8 ? c->e8[0] : c->e_base[0]);
     p.e_im = (const T *)(c->halo8 ? c->e8[1] : c->e_base[1]);
+    p.ph = (const T *)(c->halo8 ? c->ph8 : c->ph_base);
+    p.mag = c->corr_abs;
     p.W = c->Wd; p.H = c->Hd; p.pitch = c->pitch;
     p.n_strips = c->n_strips2; p.n_waves = c->n_waves2; p.seg_rows = c->seg_rows2;
     p.mH = (T)c->mH; p.tH = (T)c->tH; p.mV = (T)c->mV; p.tV = (T)c->tV;
@@ -292,18 +320,23 @@ int launch_step2(tsmi_ctx *c) {
     p.c_nl = (T)c->c_nl_s; p.c_lhy = (T)c->c_lhy_s;
     const int blocks = (c->n_waves2 + kBlock / kWave - 1) / (kBlock / kWave);
     if (int st = time_mark(c, 1, 0)) return st;
-    hipLaunchKernelGGL((step2_kernel<T, VEC, LHY, MODE>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
+    hipLaunchKernelGGL((step2_kernel<T, VEC, LHY, MODE, PH>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
     TSMI_HIP(hipGetLastError());
     if (int st = time_mark(c, 1, 1)) return st;
     c->cur = nb;
     return TSMI_OK;
 }
 
+template <typename T, int VEC, bool LHY, bool PH>
+int step2_form(tsmi_ctx *c) {
+    if (c->mode == 0) return launch_step2<T, VEC, LHY, 0, PH>(c);
+    if (c->mode == 1) return launch_step2<T, VEC, LHY, 1, PH>(c);
+    return launch_step2<T, VEC, LHY, -1, PH>(c);
+}
+
 template <typename T, int VEC, bool LHY>
 int step2_mode(tsmi_ctx *c) {
-    if (c->mode == 0) return launch_step2<T, VEC, LHY, 0>(c);
-    if (c->mode == 1) return launch_step2<T, VEC, LHY, 1>(c);
-    return launch_step2<T, VEC, LHY, -1>(c);
+    return c->ph_mode ? step2_form<T, VEC, LHY, true>(c) : step2_form<T, VEC, LHY, false>(c);
 }
 
 int step2_once(tsmi_ctx *c) {
@@ -314,7 +347,8 @@ int step2_once(tsmi_ctx *c) {
 
 template <typename T, int VEC, bool LHY>
 hipError_t occupancy2_of(int *blocks) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, step2_kernel<T, VEC, LHY, 0>, kBlock, 0);
+    // sized for the phase form (every E of unit modulus), the one the steps run
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, step2_kernel<T, VEC, LHY, 0, true>, kBlock, 0);
 }
 
 hipError_t occupancy2_blocks(int precision, bool lhy, int *blocks) {
@@ -322,7 +356,7 @@ hipError_t occupancy2_blocks(int precision, bool lhy, int *blocks) {
     return lhy ? occupancy2_of<double, 2, true>(blocks) : occupancy2_of<double, 2, false>(blocks);
 }
 
-template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
+template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH>
 int launch_step(tsmi_ctx *c, int exchange) {
     StepParams<T> p;
     std::memset(&p, 0, sizeof(p));
@@ -333,6 +367,8 @@ int launch_step(tsmi_ctx *c, int exchange) {
     p.out_im = (T *)c->psi_im[nb];
     p.e_re = (const T *)c->e_re;
     p.e_im = (const T *)c->e_im;
+    p.ph = (const T *)c->ph;
+    p.mag = IMAG ? 1.0 : c->corr_abs;
     p.W = c->W; p.H = c->H; p.pitch = c->pitch;
     p.n_strips = c->n_strips; p.n_waves = c->n_waves; p.seg_rows = c->seg_rows;
     p.mH = (T)c->mH; p.tH = (T)c->tH; p.mV = (T)c->mV; p.tV = (T)c->tV;
@@ -350,7 +386,7 @@ int launch_step(tsmi_ctx *c, int exchange) {
     // the grid covers every (strip, segment) pair exactly once
     const int blocks = (c->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
     if (int st = time_mark(c, 0, 0)) return st;
-    hipLaunchKernelGGL((step_kernel<T, IMAG, VEC, LHY, MODE>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
+    hipLaunchKernelGGL((step_kernel<T, IMAG, VEC, LHY, MODE, PH>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
     TSMI_HIP(hipGetLastError());
     if (int st = time_mark(c, 0, 1)) return st;
     c->cur = nb;
@@ -367,7 +403,7 @@ int launch_step(tsmi_ctx *c, int exchange) {
 
 template <typename T, bool IMAG, int VEC, bool LHY>
 hipError_t occupancy_of(int *blocks) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, step_kernel<T, IMAG, VEC, LHY, 0>, kBlock, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, step_kernel<T, IMAG, VEC, LHY, 0, true>, kBlock, 0);
 }
 
 hipError_t occupancy_blocks(int precision, bool imag, bool lhy, int *blocks) {
@@ -379,15 +415,21 @@ hipError_t occupancy_blocks(int precision, bool imag, bool lhy, int *blocks) {
     return lhy ? occupancy_of<double, false, 2, true>(blocks) : occupancy_of<double, false, 2, false>(blocks);
 }
 
+template <typename T, bool IMAG, int VEC, bool LHY, bool PH>
+int step_form(tsmi_ctx *c, int exchange) {
+    if constexpr (IMAG) {
+        return launch_step<T, IMAG, VEC, LHY, 0, PH>(c, exchange);   // form A on both axes
+    } else {
+        if (c->mode == 0) return launch_step<T, IMAG, VEC, LHY, 0, PH>(c, exchange);
+        if (c->mode == 1) return launch_step<T, IMAG, VEC, LHY, 1, PH>(c, exchange);
+        return launch_step<T, IMAG, VEC, LHY, -1, PH>(c, exchange);
+    }
+}
+
 template <typename T, bool IMAG, int VEC, bool LHY>
 int step_mode(tsmi_ctx *c, int exchange) {
-    if constexpr (IMAG) {
-        return launch_step<T, IMAG, VEC, LHY, 0>(c, exchange);   // form A on both axes
-    } else {
-        if (c->mode == 0) return launch_step<T, IMAG, VEC, LHY, 0>(c, exchange);
-        if (c->mode == 1) return launch_step<T, IMAG, VEC, LHY, 1>(c, exchange);
-        return launch_step<T, IMAG, VEC, LHY, -1>(c, exchange);
-    }
+    return c->ph_mode ? step_form<T, IMAG, VEC, LHY, true>(c, exchange)
+                      : step_form<T, IMAG, VEC, LHY, false>(c, exchange);
 }
 
 template <typename T, int VEC>
@@ -443,7 +485,9 @@ int wrap_exp(tsmi_ctx *c) {
     const size_t pb = plane_bytes(c);
     TSMI_HIP(hipMemcpyAsync(c->e8[0], c->e_base[0], pb, hipMemcpyDeviceToDevice, c->stream));
     TSMI_HIP(hipMemcpyAsync(c->e8[1], c->e_base[1], pb, hipMemcpyDeviceToDevice, c->stream));
-    return exchange_planes(c, c->e8[0], c->e8[1]);
+    TSMI_HIP(hipMemcpyAsync(c->ph8, c->ph_base, pb, hipMemcpyDeviceToDevice, c->stream));
+    if (int st = exchange_planes(c, c->e8[0], c->e8[1])) return st;
+    return exchange_planes(c, c->ph8, c->ph8);   // one plane: the copy runs twice, to the same values
 }
 
 // One reverse step: (s_in, g_in) -> (s_out, g_out), gradient into c->grad.
@@ -686,6 +730,7 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     struct { void **p; size_t n; } allocs[] = {
         {&c->psi_base[0][0], pb}, {&c->psi_base[0][1], pb}, {&c->psi_base[1][0], pb}, {&c->psi_base[1][1], pb},
         {&c->e_base[0], pb}, {&c->e_base[1], pb}, {&c->e8[0], halo8 ? pb : 0}, {&c->e8[1], halo8 ? pb : 0},
+        {&c->ph_base, pb}, {&c->ph8, halo8 ? pb : 0}, {(void **)&c->flag, 64},
         {(void **)&c->v_base, pd},
         {(void **)&c->xc, (size_t)c->W * sizeof(double)}, {(void **)&c->yc, (size_t)c->H * sizeof(double)},
         {(void **)&c->div, 2 * sizeof(double)}, {(void **)&c->partials, (size_t)c->n_waves * sizeof(double)},
@@ -705,6 +750,7 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     }
     c->e_re = (char *)c->e_base[0] + ob;
     c->e_im = (char *)c->e_base[1] + ob;
+    c->ph = (char *)c->ph_base + ob;
     c->v = c->v_base + c->off_el;
     std::vector<double> xs, ys;
     map_axis(d.periodic_x ? -d.halo_x : 0, d.delta_x, d.dim_x, d.length_x, c->W, xs);
@@ -771,11 +817,13 @@ int tsmi_set_potential_exp_device(tsmi_ctx *c, const double *v) {
     const int imag = c->d.imag_time ? 1 : 0;
     if (c->esz == 4)
         hipLaunchKernelGGL(exp_from_potential<float>, dim3(2048), dim3(256), 0, c->stream, v, dt, imag, c->corr_re,
-                           c->corr_im, (float *)c->e_re, (float *)c->e_im, c->v, c->W, c->H, c->pitch);
+                           c->corr_im, (float *)c->e_re, (float *)c->e_im, (float *)c->ph, c->v, c->W, c->H, c->pitch);
     else
         hipLaunchKernelGGL(exp_from_potential<double>, dim3(2048), dim3(256), 0, c->stream, v, dt, imag, c->corr_re,
-                           c->corr_im, (double *)c->e_re, (double *)c->e_im, c->v, c->W, c->H, c->pitch);
+                           c->corr_im, (double *)c->e_re, (double *)c->e_im, (double *)c->ph, c->v, c->W, c->H,
+                           c->pitch);
     TSMI_HIP(hipGetLastError());
+    c->ph_mode = true;   // |E| = 1 (real time), E > 0 (imaginary time) by construction
     if (int we = wrap_exp(c)) return we;
     c->has_e = true;
     c->has_v = true;
