@@ -29,6 +29,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 BYTES_PER_SU = {32: 24, 64: 48}   # psi read + psi write + E read per inner site (real time)
+# What the kernel reads and writes per inner site and step when E is a pure phase
+# (the case here): E as one phase plane instead of two planes
+KERNEL_BYTES_PER_SU = {32: 20, 64: 40}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_8192_traffic.json")
 
 
@@ -363,7 +366,10 @@ def main():
                      "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": r["kernel"], "kernel_us_per_launch": r["kernel_us"], "launches": r["launches"],
                      "bytes_per_site_update": BYTES_PER_SU[main_prec], "sweeps_per_launch": r["sweeps"],
-                     "algorithmic_bytes_per_launch": r["sweeps"] * BYTES_PER_SU[main_prec] * sites},
+                     "algorithmic_bytes_per_launch": r["sweeps"] * BYTES_PER_SU[main_prec] * sites,
+                     "e_form": "phase plane (one plane of E's two)",
+                     "kernel_bytes_per_site_update": KERNEL_BYTES_PER_SU[main_prec],
+                     "kernel_bytes_achieved": r["achieved"] * KERNEL_BYTES_PER_SU[main_prec] / BYTES_PER_SU[main_prec]},
         "cpu_baseline": cpu,
     }
     if 64 in results and main_prec != 64:
