@@ -29,6 +29,8 @@ struct PairParams {
     const T *in_re[2], *in_im[2];
     T *out_re[2], *out_im[2];
     const T *e_re[2], *e_im[2];      // E of each component (e_im unused in imaginary time)
+    const T *ph[2];                  // one-plane form of each E (see StepParams)
+    double mag[2];                   // |corr| of each component (real time, phase form)
     int W, H, pitch;
     int n_strips, n_waves, seg_rows;
     T mH[2], tH[2], mV[2], tV[2], unH[2], unV[2];   // scaled-frame pair factors per component
@@ -52,13 +54,15 @@ struct PairParams {
 // P of one component (src/cpucartesian.cpp:75-92 / :115-126): the phase or decay
 // sees its own density (scaled frame, c already rescaled) and the other
 // component's start-of-step density nb.
-template <bool IMAG, typename T, int VEC>
-__device__ __forceinline__ void ppass_pair(Row<T, VEC> &r, const Row<T, VEC> &e, const T (&nb)[VEC], T c, T cab) {
+template <bool IMAG, bool PH, typename T, int VEC>
+__device__ __forceinline__ void ppass_pair(Row<T, VEC> &r, const Row<T, VEC> &e, const T (&nb)[VEC], T c, T cab,
+                                           double mag) {
     T ang[VEC];
 #pragma unroll
     for (int k = 0; k < VEC; ++k) {
         const T n2 = r.re[k] * r.re[k] + r.im[k] * r.im[k];
         ang[k] = c * n2 + cab * nb[k];
+        if (PH) ang[k] += e.re[k];
     }
     if (!IMAG) {
         T sn[VEC], cs[VEC];
@@ -66,18 +70,26 @@ __device__ __forceinline__ void ppass_pair(Row<T, VEC> &r, const Row<T, VEC> &e,
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
             const T pr = r.re[k], pi = r.im[k];
-            const T r1 = e.re[k] * pr - e.im[k] * pi;
-            const T i1 = e.re[k] * pi + e.im[k] * pr;
-            r.re[k] = cs[k] * r1 + sn[k] * i1;
-            r.im[k] = cs[k] * i1 - sn[k] * r1;
+            if (PH) {
+                T c2, s2;
+                phase_factor<T>(cs[k], sn[k], mag, c2, s2);
+                r.re[k] = c2 * pr + s2 * pi;
+                r.im[k] = c2 * pi - s2 * pr;
+            } else {
+                const T r1 = e.re[k] * pr - e.im[k] * pi;
+                const T i1 = e.re[k] * pi + e.im[k] * pr;
+                r.re[k] = cs[k] * r1 + sn[k] * i1;
+                r.im[k] = cs[k] * i1 - sn[k] * r1;
+            }
         }
     } else {
         T f[VEC];
         expneg_row<T, VEC>(ang, f);
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
-            r.re[k] = f[k] * e.re[k] * r.re[k];
-            r.im[k] = f[k] * e.re[k] * r.im[k];
+            const T g = PH ? f[k] : f[k] * e.re[k];
+            r.re[k] = g * r.re[k];
+            r.im[k] = g * r.im[k];
         }
     }
 }
@@ -89,10 +101,11 @@ struct Pipe4 {
 
 // One beat of one component's pipeline (the general form of step_wave's beat):
 // input row t enters, row t-4 leaves in o4.
-template <int TP, bool IMAG, typename T, int VEC>
+template <int TP, bool IMAG, bool PH, typename T, int VEC>
 __device__ __forceinline__ void pair_pipe(Pipe4<T, VEC> &q, Row<T, VEC> &in, const Row<T, VEC> &e,
                                           const T (&nb)[VEC], Row<T, VEC> &o4, bool v0, bool v1, bool v2, bool v3,
-                                          bool v4, const HCoef<T, VEC> &hc, T mV, T tV, T unV, T c, T cab) {
+                                          bool v4, const HCoef<T, VEC> &hc, T mV, T tV, T unV, T c, T cab,
+                                          double mag) {
     const T a1 = (v0 && v1) ? mV : unV, b1 = (v0 && v1) ? tV : T(0);
     const T a2 = (v1 && v2) ? mV : unV, b2 = (v1 && v2) ? tV : T(0);
     const T a3 = (v2 && v3) ? mV : unV, b3 = (v2 && v3) ? tV : T(0);
@@ -107,7 +120,7 @@ __device__ __forceinline__ void pair_pipe(Pipe4<T, VEC> &q, Row<T, VEC> &in, con
     Row<T, VEC> o2 = q.p2;
     q.p2 = o1;
     hpass<(TP + 1) & 1, IMAG>(o2, hc);
-    ppass_pair<IMAG>(o2, e, nb, c, cab);
+    ppass_pair<IMAG, PH>(o2, e, nb, c, cab, mag);
     hpass<(TP + 1) & 1, IMAG>(o2, hc);
     // stage 3: V1, then H0 on row t-3
     vpair<TP & 1, IMAG>(q.p3, o2, a3, b3);
@@ -132,7 +145,7 @@ __device__ __forceinline__ void neumaier(double &s, double &c, double v) {
     s = t;
 }
 
-template <typename T, bool IMAG, int VEC>
+template <typename T, bool IMAG, int VEC, bool PH>
 __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave, const int lane) {
     constexpr int SW = kWave * VEC - 2 * kApron;
     const int strip = wave % p.n_strips;
@@ -162,10 +175,14 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
     const __amdgpu_buffer_rsrc_t ri0i = make_rsrc(p.in_im[0] + seg0, seg_bytes);
     const __amdgpu_buffer_rsrc_t ri1r = make_rsrc(p.in_re[1] + seg0, seg_bytes);
     const __amdgpu_buffer_rsrc_t ri1i = make_rsrc(p.in_im[1] + seg0, seg_bytes);
-    const __amdgpu_buffer_rsrc_t re0r = make_rsrc(p.e_re[0] + seg0, seg_bytes);
-    const __amdgpu_buffer_rsrc_t re0i = make_rsrc(IMAG ? p.e_re[0] + seg0 : p.e_im[0] + seg0, seg_bytes);
-    const __amdgpu_buffer_rsrc_t re1r = make_rsrc(p.e_re[1] + seg0, seg_bytes);
-    const __amdgpu_buffer_rsrc_t re1i = make_rsrc(IMAG ? p.e_re[1] + seg0 : p.e_im[1] + seg0, seg_bytes);
+    // E of each component, or its one-plane form (the second plane exists only
+    // for complex E in the E form)
+    const T *e0 = PH ? p.ph[0] : p.e_re[0], *e1 = PH ? p.ph[1] : p.e_re[1];
+    const __amdgpu_buffer_rsrc_t re0r = make_rsrc(e0 + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t re0i = make_rsrc((IMAG || PH) ? e0 + seg0 : p.e_im[0] + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t re1r = make_rsrc(e1 + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t re1i = make_rsrc((IMAG || PH) ? e1 + seg0 : p.e_im[1] + seg0, seg_bytes);
+    const double mag0 = p.mag[0], mag1 = p.mag[1];
     const __amdgpu_buffer_rsrc_t ro0r = make_rsrc(p.out_re[0] + seg0, seg_bytes);
     const __amdgpu_buffer_rsrc_t ro0i = make_rsrc(p.out_im[0] + seg0, seg_bytes);
     const __amdgpu_buffer_rsrc_t ro1r = make_rsrc(p.out_re[1] + seg0, seg_bytes);
@@ -203,7 +220,7 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
         buf_load<T, VEC>(re0r, v_ld, so, a.re);
         buf_load<T, VEC>(re1r, v_ld, so, b.re);
-        if (!IMAG) {
+        if (!IMAG && !PH) {
             buf_load<T, VEC>(re0i, v_ld, so, a.im);
             buf_load<T, VEC>(re1i, v_ld, so, b.im);
         }
@@ -266,8 +283,8 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
         density(ia, ta);
         density(ib, tb);
         Row<T, VEC> oa, ob;
-        pair_pipe<TP, IMAG>(qa, ia, ea, nb, oa, v0, v1, v2, v3, v4, hc0, mV0, tV0, unV0, c0, cab);
-        pair_pipe<TP, IMAG>(qb, ib, eb, na, ob, v0, v1, v2, v3, v4, hc1, mV1, tV1, unV1, c1, cab);
+        pair_pipe<TP, IMAG, PH>(qa, ia, ea, nb, oa, v0, v1, v2, v3, v4, hc0, mV0, tV0, unV0, c0, cab, mag0);
+        pair_pipe<TP, IMAG, PH>(qb, ib, eb, na, ob, v0, v1, v2, v3, v4, hc1, mV1, tV1, unV1, c1, cab, mag1);
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
             na[k] = ta[k];
@@ -332,12 +349,12 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
     }
 }
 
-template <typename T, bool IMAG, int VEC>
-__global__ void __launch_bounds__(256) pair_kernel(PairParams<T> p) {
+template <typename T, bool IMAG, int VEC, bool PH>
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) pair_kernel(PairParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
-    pair_wave<T, IMAG, VEC>(p, wave, lane);
+    pair_wave<T, IMAG, VEC, PH>(p, wave, lane);
 }
 
 // ---------------------------------------------------------------------------

@@ -361,6 +361,24 @@ __device__ __forceinline__ void expneg_row(const T (&x)[VEC], T (&e)[VEC]) {
     }
 }
 
+// mag (cos, sin) of the phase form, rounded to the kernel precision.  In FP32,
+// |corr| differs from 1 by less than half an ulp near 1, so cos * |corr| rounded
+// alone would drop it at almost every site (a norm bias of ~6e-9 per step);
+// renormalising (cos, sin) in FP64 first (first order in cos^2 + sin^2 - 1)
+// dithers that rounding and leaves no bias above ~1e-10 per step.
+template <typename T>
+__device__ __forceinline__ void phase_factor(T cs, T sn, double mag, T &c2, T &s2) {
+    if constexpr (sizeof(T) == 4) {
+        const double cd = cs, sd = sn;
+        const double kk = mag * (1.5 - 0.5 * (cd * cd + sd * sd));
+        c2 = T(cd * kk);
+        s2 = T(sd * kk);
+    } else {
+        c2 = cs * mag;
+        s2 = sn * mag;
+    }
+}
+
 // P on one row (src/cpucartesian.cpp:93-109 / :127-137).  PH: e.re holds the
 // phase phi (real time) or the exponent lambda (imaginary time) of E corr, which
 // joins the nonlinear angle: one sincos (or exp) gives both factors, and |E|
@@ -383,20 +401,7 @@ __device__ __forceinline__ void ppass(Row<T, VEC> &r, const Row<T, VEC> &e, T c,
             const T pr = r.re[k], pi = r.im[k];
             if (PH) {
                 T c2, s2;
-                if constexpr (sizeof(T) == 4) {
-                    // |corr| differs from 1 by less than half an FP32 ulp near 1, so
-                    // cos * |corr| rounded alone would drop it at almost every site
-                    // (a norm bias of ~6e-9 per step); renormalising (cos, sin) in
-                    // FP64 first (first order in cos^2 + sin^2 - 1) dithers that
-                    // rounding and leaves no bias above ~1e-10 per step
-                    const double cd = cs[k], sd = sn[k];
-                    const double kk = mag * (1.5 - 0.5 * (cd * cd + sd * sd));
-                    c2 = T(cd * kk);
-                    s2 = T(sd * kk);
-                } else {
-                    c2 = cs[k] * mag;
-                    s2 = sn[k] * mag;
-                }
+                phase_factor<T>(cs[k], sn[k], mag, c2, s2);
                 r.re[k] = c2 * pr + s2 * pi;
                 r.im[k] = c2 * pi - s2 * pr;
             } else {

@@ -95,7 +95,7 @@ double *norms(tsmi_pair *p) { return p->dv + 4; }
 
 // One step of both components; in real time the Rabi rotation by var leaves the
 // pipelines with the rows when rotate.
-template <typename T, bool IMAG>
+template <typename T, bool IMAG, bool PH>
 int launch_pair_step(tsmi_pair *p, double var, bool rotate) {
     PairParams<T> q;
     std::memset(&q, 0, sizeof(q));
@@ -108,6 +108,8 @@ int launch_pair_step(tsmi_pair *p, double var, bool rotate) {
         q.out_im[i] = (T *)c->psi_im[nb];
         q.e_re[i] = (const T *)c->e_re;
         q.e_im[i] = (const T *)c->e_im;
+        q.ph[i] = (const T *)c->ph;
+        q.mag[i] = IMAG ? 1.0 : c->corr_abs;
         q.mH[i] = (T)c->mH; q.tH[i] = (T)c->tH; q.mV[i] = (T)c->mV; q.tV[i] = (T)c->tV;
         q.unH[i] = (T)c->unH; q.unV[i] = (T)c->unV;
         q.c_self[i] = (T)c->c_nl_s;
@@ -135,7 +137,7 @@ int launch_pair_step(tsmi_pair *p, double var, bool rotate) {
     }
     constexpr int VEC = sizeof(T) == 4 ? kVecPair32 : 2;
     const int blocks = (p->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
-    hipLaunchKernelGGL((pair_kernel<T, IMAG, VEC>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
+    hipLaunchKernelGGL((pair_kernel<T, IMAG, VEC, PH>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
     TSMI_HIPP(hipGetLastError());
     p->c[0]->cur = nb;
     p->c[1]->cur = nb;
@@ -197,10 +199,16 @@ int rabi_pass(tsmi_pair *p, double var, bool rotate, bool normalize) {
     return p->esz == 4 ? launch_rabi<float>(p, var, rotate, normalize) : launch_rabi<double>(p, var, rotate, normalize);
 }
 
+template <typename T, bool PH>
+int pair_step_form(tsmi_pair *p, double var, bool rotate) {
+    return p->imag ? launch_pair_step<T, true, PH>(p, var, rotate) : launch_pair_step<T, false, PH>(p, var, rotate);
+}
+
+// the one-plane form of E when both components have it
 int pair_step(tsmi_pair *p, double var, bool rotate) {
-    if (p->esz == 4)
-        return p->imag ? launch_pair_step<float, true>(p, var, rotate) : launch_pair_step<float, false>(p, var, rotate);
-    return p->imag ? launch_pair_step<double, true>(p, var, rotate) : launch_pair_step<double, false>(p, var, rotate);
+    const bool ph = p->c[0]->ph_mode && p->c[1]->ph_mode;
+    if (p->esz == 4) return ph ? pair_step_form<float, true>(p, var, rotate) : pair_step_form<float, false>(p, var, rotate);
+    return ph ? pair_step_form<double, true>(p, var, rotate) : pair_step_form<double, false>(p, var, rotate);
 }
 
 // the periodic exchange of both components' current tiles
@@ -224,7 +232,7 @@ int check_ready(tsmi_pair *p, const char *who) {
 template <typename T, bool IMAG>
 hipError_t pair_occupancy(int *blocks) {
     constexpr int VEC = sizeof(T) == 4 ? kVecPair32 : 2;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, pair_kernel<T, IMAG, VEC>, kBlock, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, pair_kernel<T, IMAG, VEC, true>, kBlock, 0);
 }
 
 hipError_t occupancy_of(int esz, bool imag, int *blocks) {
