@@ -109,3 +109,27 @@ def test_batch_rejects_mixed_precision():
     h = C.c_void_p()
     assert L.tsmi_batch_create(descs, 2, C.byref(h)) == 4
     assert b"same precision" in L.tsmi_last_error()
+
+
+def test_batch_non_unit_e_keeps_e_form(oracle):
+    """One lattice with an E table that is not a pure phase puts the whole batch
+    on the E form; every lattice still matches the oracle."""
+    import ctypes as C
+    from tsmi.batch import HipBatch
+    wls = workloads(MIXED[:4])
+    es = oracle_e(oracle, wls)
+    er, ei = es[1]
+    damp = 0.999 - 0.01 * np.linspace(0.0, 1.0, er.shape[1])[None, :]
+    es[1] = (er * damp, ei * damp)
+    refs = []
+    for wl, (a, b) in zip(wls, es):
+        d = oracle.make_desc(wl)
+        re = np.ascontiguousarray(wl.psi.real).copy()
+        im = np.ascontiguousarray(wl.psi.imag).copy()
+        oracle.lib().ts_oracle_evolve(C.byref(d), 9, oracle._p(re), oracle._p(im),
+                                      oracle._p(np.ascontiguousarray(a)), oracle._p(np.ascontiguousarray(b)))
+        refs.append(re + 1j * im)
+    with HipBatch(wls, precision=64, e_planes=es) as b:
+        b.evolve(9)
+        for i in range(len(wls)):
+            assert rel_l2(b.state(i), refs[i]) < 1e-12, i

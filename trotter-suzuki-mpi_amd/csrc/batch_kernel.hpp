@@ -19,7 +19,7 @@
 
 namespace tsmi {
 
-template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
+template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH>
 __global__ void __launch_bounds__(256) batch_step_kernel(const StepParams<T> *__restrict__ params,
                                                          const int2 *__restrict__ items, int n_items) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -27,11 +27,11 @@ __global__ void __launch_bounds__(256) batch_step_kernel(const StepParams<T> *__
     const int stride = __builtin_amdgcn_readfirstlane(gridDim.x * (blockDim.x / kWave));
     for (int it = first; it < n_items; it += stride) {
         const int2 e = items[it];
-        step_wave<T, IMAG, VEC, LHY, MODE>(params[e.x], e.y, lane);
+        step_wave<T, IMAG, VEC, LHY, MODE, PH>(params[e.x], e.y, lane);
     }
 }
 
-template <typename T, int VEC, bool LHY, int MODE>
+template <typename T, int VEC, bool LHY, int MODE, bool PH>
 __global__ void __launch_bounds__(256) batch_step2_kernel(const StepParams<T> *__restrict__ params,
                                                           const int2 *__restrict__ items, int n_items) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(256) batch_step2_kernel(const StepParams<T> *_
     const int stride = __builtin_amdgcn_readfirstlane(gridDim.x * (blockDim.x / kWave));
     for (int it = first; it < n_items; it += stride) {
         const int2 e = items[it];
-        step2_wave<T, VEC, LHY, MODE>(params[e.x], e.y, lane);
+        step2_wave<T, VEC, LHY, MODE, PH>(params[e.x], e.y, lane);
     }
 }
 

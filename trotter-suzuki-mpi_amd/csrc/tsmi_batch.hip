@@ -70,6 +70,8 @@ struct Lattice {
     size_t psi[2][2] = {{0, 0}, {0, 0}};      // arena offsets of the views: [buffer][re/im]
     size_t e[2] = {0, 0};                     // view of E
     size_t e8[2] = {0, 0};                    // allocation of the E the fused passes read
+    size_t ph = 0, ph8 = 0;                   // the one-plane form of E: view, and what the fused passes read
+    bool ph_ok = false;                       // E has that form (|E| = 1, or E > 0)
     size_t partials = 0, div = 0, counter = 0;
     int group() const { return fused ? 1 : 0; }
 };
@@ -99,7 +101,8 @@ struct tsmi_batch {
     void *exch[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [group][buffer holding the state]
     int n_exch[2] = {0, 0}, exch_chunks[2] = {1, 1};
     int cur[2] = {0, 0};                      // buffer holding the states, per group
-    int grid = 0, grid2 = 0;
+    bool ph = false;                          // every lattice's E has the one-plane form
+    int grid[2] = {0, 0}, grid2[2] = {0, 0};   // persistent grids of the E form [0] and the phase form [1]
     hipStream_t own = nullptr, stream = nullptr, side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     std::map<std::tuple<int, int, int>, GraphRec> graphs;
@@ -162,6 +165,8 @@ StepParams<T> params_of(const tsmi_batch *b, const Lattice &L, int parity) {
     p.out_im = (T *)(a + L.psi[1 - parity][1]);
     p.e_re = (const T *)(a + L.e[0]);
     p.e_im = (const T *)(a + L.e[1]);
+    p.ph = (const T *)(a + L.ph);
+    p.mag = b->imag ? 1.0 : L.k.corr_abs;
     p.W = L.W; p.H = L.H; p.pitch = L.pitch;
     p.n_strips = L.n_strips; p.n_waves = L.n_waves; p.seg_rows = L.seg_rows;
     set_coefs(p, L);
@@ -191,6 +196,8 @@ StepParams<T> params2_of(const tsmi_batch *b, const Lattice &L, int parity) {
     p.out_im = (T *)(a + L.psi[1 - parity][1] - L.shift);
     p.e_re = (const T *)(a + L.e8[0]);
     p.e_im = (const T *)(a + L.e8[1]);
+    p.ph = (const T *)(a + L.ph8);
+    p.mag = L.k.corr_abs;
     p.W = L.Wd; p.H = L.Hd; p.pitch = L.pitch;
     p.n_strips = L.n_strips2; p.n_waves = L.n_waves2; p.seg_rows = L.seg_rows2;
     set_coefs(p, L);
@@ -291,8 +298,12 @@ int upload_exchange(tsmi_batch *b) {
 
 template <typename T, bool IMAG, int VEC, bool LHY, int MODE>
 int launch_single(tsmi_batch *b, hipStream_t s, int list, int parity) {
-    hipLaunchKernelGGL((batch_step_kernel<T, IMAG, VEC, LHY, MODE>), dim3(b->grid), dim3(kBlock), 0, s,
-                       (const StepParams<T> *)b->params[parity], (const int2 *)b->items[list], b->n_items[list]);
+    if (b->ph)
+        hipLaunchKernelGGL((batch_step_kernel<T, IMAG, VEC, LHY, MODE, true>), dim3(b->grid[1]), dim3(kBlock), 0, s,
+                           (const StepParams<T> *)b->params[parity], (const int2 *)b->items[list], b->n_items[list]);
+    else
+        hipLaunchKernelGGL((batch_step_kernel<T, IMAG, VEC, LHY, MODE, false>), dim3(b->grid[0]), dim3(kBlock), 0, s,
+                           (const StepParams<T> *)b->params[parity], (const int2 *)b->items[list], b->n_items[list]);
     TSMI_HIPB(hipGetLastError());
     return TSMI_OK;
 }
@@ -319,8 +330,12 @@ int single_typed(tsmi_batch *b, hipStream_t s, int list, int parity) {
 
 template <typename T, bool LHY, int MODE>
 int launch_fused(tsmi_batch *b, hipStream_t s, int parity) {
-    hipLaunchKernelGGL((batch_step2_kernel<T, kVecF, LHY, MODE>), dim3(b->grid2), dim3(kBlock), 0, s,
-                       (const StepParams<T> *)b->params2[parity], (const int2 *)b->items[2], b->n_items[2]);
+    if (b->ph)
+        hipLaunchKernelGGL((batch_step2_kernel<T, kVecF, LHY, MODE, true>), dim3(b->grid2[1]), dim3(kBlock), 0, s,
+                           (const StepParams<T> *)b->params2[parity], (const int2 *)b->items[2], b->n_items[2]);
+    else
+        hipLaunchKernelGGL((batch_step2_kernel<T, kVecF, LHY, MODE, false>), dim3(b->grid2[0]), dim3(kBlock), 0, s,
+                           (const StepParams<T> *)b->params2[parity], (const int2 *)b->items[2], b->n_items[2]);
     TSMI_HIPB(hipGetLastError());
     return TSMI_OK;
 }
@@ -394,27 +409,31 @@ int batch_run(tsmi_batch *b, int n, int *cur) {
     return st;
 }
 
-template <typename T, bool IMAG, int VEC, bool LHY>
+// Resident blocks per CU of the persistent kernels, per E form (a persistent
+// grid larger than what is resident would leave items to late waves).
+template <typename T, bool IMAG, int VEC, bool LHY, bool PH>
 hipError_t single_occupancy(int *blocks) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, batch_step_kernel<T, IMAG, VEC, LHY, 0>, kBlock, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, batch_step_kernel<T, IMAG, VEC, LHY, 0, PH>, kBlock, 0);
 }
 
+template <bool PH>
 hipError_t occupancy_of(const tsmi_batch *b, int *blocks) {
     if (b->esz == 4) {
-        if (b->imag) return b->lhy ? single_occupancy<float, true, 4, true>(blocks) : single_occupancy<float, true, 4, false>(blocks);
-        return b->lhy ? single_occupancy<float, false, 4, true>(blocks) : single_occupancy<float, false, 4, false>(blocks);
+        if (b->imag) return b->lhy ? single_occupancy<float, true, 4, true, PH>(blocks) : single_occupancy<float, true, 4, false, PH>(blocks);
+        return b->lhy ? single_occupancy<float, false, 4, true, PH>(blocks) : single_occupancy<float, false, 4, false, PH>(blocks);
     }
-    if (b->imag) return b->lhy ? single_occupancy<double, true, 2, true>(blocks) : single_occupancy<double, true, 2, false>(blocks);
-    return b->lhy ? single_occupancy<double, false, 2, true>(blocks) : single_occupancy<double, false, 2, false>(blocks);
+    if (b->imag) return b->lhy ? single_occupancy<double, true, 2, true, PH>(blocks) : single_occupancy<double, true, 2, false, PH>(blocks);
+    return b->lhy ? single_occupancy<double, false, 2, true, PH>(blocks) : single_occupancy<double, false, 2, false, PH>(blocks);
 }
 
+template <bool PH>
 hipError_t occupancy2_of(const tsmi_batch *b, int *blocks) {
     auto q = [&](const void *k) { return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k, kBlock, 0); };
     if (b->esz == 4)
-        return b->lhy ? q((const void *)batch_step2_kernel<float, kVecF, true, 0>)
-                      : q((const void *)batch_step2_kernel<float, kVecF, false, 0>);
-    return b->lhy ? q((const void *)batch_step2_kernel<double, kVecF, true, 0>)
-                  : q((const void *)batch_step2_kernel<double, kVecF, false, 0>);
+        return b->lhy ? q((const void *)batch_step2_kernel<float, kVecF, true, 0, PH>)
+                      : q((const void *)batch_step2_kernel<float, kVecF, false, 0, PH>);
+    return b->lhy ? q((const void *)batch_step2_kernel<double, kVecF, true, 0, PH>)
+                  : q((const void *)batch_step2_kernel<double, kVecF, false, 0, PH>);
 }
 
 Lattice *lattice_at(tsmi_batch *b, int i, const char *who) {
@@ -460,18 +479,23 @@ int wrap_exp(tsmi_batch *b, const Lattice &L) {
     for (int c = 0; c < 2; ++c)
         TSMI_HIPB(hipMemcpyAsync(b->arena + L.e8[c], b->arena + L.e[c] - L.shift, pb, hipMemcpyDeviceToDevice,
                                  b->stream));
+    TSMI_HIPB(hipMemcpyAsync(b->arena + L.ph8, b->arena + L.ph - L.shift, pb, hipMemcpyDeviceToDevice, b->stream));
     const int hx = 2 * L.d.halo_x, hy = 2 * L.d.halo_y;
     const long cells = exchange_cells(L.Wd, L.Hd, 1, 1, hx, hy);
     const int eb = (int)std::min<long>(1024, (cells + 255) / 256);
-    if (b->esz == 4)
-        hipLaunchKernelGGL((halo_exchange_kernel<float>), dim3(eb), dim3(256), 0, b->stream,
-                           (float *)(b->arena + L.e8[0]), (float *)(b->arena + L.e8[1]), L.Wd, L.Hd, L.pitch, 1, 1,
-                           hx, hy);
-    else
-        hipLaunchKernelGGL((halo_exchange_kernel<double>), dim3(eb), dim3(256), 0, b->stream,
-                           (double *)(b->arena + L.e8[0]), (double *)(b->arena + L.e8[1]), L.Wd, L.Hd, L.pitch, 1,
-                           1, hx, hy);
-    TSMI_HIPB(hipGetLastError());
+    // E's two planes, then the one-plane form (given twice: the copy runs twice, to the same values)
+    const size_t planes[2][2] = {{L.e8[0], L.e8[1]}, {L.ph8, L.ph8}};
+    for (const auto &pl : planes) {
+        if (b->esz == 4)
+            hipLaunchKernelGGL((halo_exchange_kernel<float>), dim3(eb), dim3(256), 0, b->stream,
+                               (float *)(b->arena + pl[0]), (float *)(b->arena + pl[1]), L.Wd, L.Hd, L.pitch, 1, 1,
+                               hx, hy);
+        else
+            hipLaunchKernelGGL((halo_exchange_kernel<double>), dim3(eb), dim3(256), 0, b->stream,
+                               (double *)(b->arena + pl[0]), (double *)(b->arena + pl[1]), L.Wd, L.Hd, L.pitch, 1,
+                               1, hx, hy);
+        TSMI_HIPB(hipGetLastError());
+    }
     TSMI_HIPB(hipStreamSynchronize(b->stream));
     return TSMI_OK;
 }
@@ -540,6 +564,9 @@ int tsmi_batch_create(const tsmi_desc *descs, int32_t n, tsmi_batch **out) {
             if (L.halo8) { L.e8[c] = off; off += pb; }
             else L.e8[c] = L.e[c];
         }
+        L.ph = off + L.shift; off += pb;
+        if (L.halo8) { L.ph8 = off; off += pb; }
+        else L.ph8 = L.ph;
         L.partials = off; off += align256((size_t)L.max_waves * sizeof(double));
         L.div = off; off += 256;
         L.counter = off; off += 256;
@@ -571,14 +598,18 @@ int tsmi_batch_create(const tsmi_desc *descs, int32_t n, tsmi_batch **out) {
     int st = b->esz == 4 ? upload_exchange<float>(b) : upload_exchange<double>(b);
     if (!st) st = upload_layout(b);
     if (st) return bail(st);
-    int dev = 0, cus = 0, per_cu = 0, per_cu2 = 0;
+    int dev = 0, cus = 0, per_cu[2] = {0, 0}, per_cu2[2] = {0, 0};
     e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = occupancy_of(b, &per_cu);
-    if (e == hipSuccess) e = occupancy2_of(b, &per_cu2);
+    if (e == hipSuccess) e = occupancy_of<false>(b, &per_cu[0]);
+    if (e == hipSuccess) e = occupancy_of<true>(b, &per_cu[1]);
+    if (e == hipSuccess) e = occupancy2_of<false>(b, &per_cu2[0]);
+    if (e == hipSuccess) e = occupancy2_of<true>(b, &per_cu2[1]);
     if (e != hipSuccess) return bail(set_error(TSMI_E_HIP, std::string("occupancy query: ") + hipGetErrorString(e)));
-    b->grid = std::max(1, cus * std::max(1, per_cu));
-    b->grid2 = std::max(1, cus * std::max(1, per_cu2));
+    for (int f = 0; f < 2; ++f) {
+        b->grid[f] = std::max(1, cus * std::max(1, per_cu[f]));
+        b->grid2[f] = std::max(1, cus * std::max(1, per_cu2[f]));
+    }
     e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return bail(set_error(TSMI_E_HIP, std::string("batch set-up: ") + hipGetErrorString(e)));
     *out = b;
@@ -610,16 +641,34 @@ int tsmi_batch_set_exp_potential(tsmi_batch *b, int32_t i, const double *e_re, c
     if (!e_re || (!b->imag && !e_im)) return set_error(TSMI_E_INVALID, "tsmi_batch_set_exp_potential: null argument");
     // E times the frame correction, in FP64, rounded once
     const size_t n = (size_t)L->W * (size_t)L->H;
-    std::vector<double> re(n), im(n, 0.0);
+    std::vector<double> re(n), im(n, 0.0), ph(n, 0.0);
+    bool ok = true;
     for (size_t k = 0; k < n; ++k) {
         const double er = e_re[k], ei = b->imag ? 0.0 : e_im[k];
         re[k] = er * L->k.corr_re - ei * L->k.corr_im;
         im[k] = er * L->k.corr_im + ei * L->k.corr_re;
+        // the one-plane form, as the single-lattice import computes it
+        if (!b->imag) {
+            ok = ok && std::fabs(er * er + ei * ei - 1.0) <= 1e-12;
+            ph[k] = -std::atan2(im[k], re[k]);
+        } else {
+            ok = ok && re[k] > 0.0;
+            ph[k] = re[k] > 0.0 ? -std::log(re[k]) : 0.0;
+        }
     }
     int st = upload_host_plane(b, *L, L->e[0], re.data());
     if (!st) st = upload_host_plane(b, *L, L->e[1], im.data());
+    if (!st) st = upload_host_plane(b, *L, L->ph, ph.data());
     if (!st) st = wrap_exp(b, *L);
-    return st;
+    if (st) return st;
+    L->ph_ok = ok;
+    bool all = true;
+    for (const Lattice &M : b->lat) all = all && M.ph_ok;
+    if (all != b->ph) {
+        b->ph = all;
+        drop_graphs(b);   // recorded with the other form's kernels
+    }
+    return TSMI_OK;
 }
 
 int tsmi_batch_evolve(tsmi_batch *b, int32_t n) {
