@@ -188,11 +188,12 @@ def batch_study(steps: int = 500, count: int = 512):
         b.evolve(steps)
         b.synchronize()
         sec = time.perf_counter() - t
-        rows = sorted({(b.info(i)["H"], b.info(i)["rows_per_segment"]) for i in range(len(wls))})
+        rows = sorted({(b.info(i)["H"], b.info(i)["steps_per_pass"], b.info(i)["rows_per_segment"])
+                       for i in range(len(wls))})
     return {"workload": f"{count} real-time lattices, sides 64..1024 (seeded), mixed closed/periodic, FP32, "
                         f"{steps} graph-captured steps", "inner_sites": sites,
             "site_updates_per_s": sites * steps / sec, "ms_per_step": sec * 1e3 / steps,
-            "tuned_ms_per_step": ms_tuned, "rows_per_segment_by_tile_height": rows, "host_build_s": build}
+            "tuned_ms_per_step": ms_tuned, "tile_height_steps_per_pass_rows": rows, "host_build_s": build}
 
 
 def adjoint_study():

@@ -152,11 +152,14 @@ int tsmi_adjoint_device(tsmi_ctx *ctx, const double *g_re, const double *g_im, d
  * tsmi_batch_evolve(n) records its launch sequence in a HIP graph the first time
  * n is used from given buffer parities and replays the graph afterwards;
  * asynchronous.  tsmi_batch_tune times the batch (`steps` steps per trial, at
- * least 6 when some lattice takes two steps per pass) for candidate segment
- * heights of each size class's dominant pass, one class (kind and tile shape) at
- * a time, keeps the fastest, restores the states and reports the tuned time per
- * step.  tsmi_batch_info: out[0..2] W, H, pitch of lattice i's tile, out[3..5]
- * strips, segments and rows per segment of its dominant pass.
+ * least 6 when some lattice can take two steps per pass, at least 24 when some
+ * lattice can take either) for candidate segment
+ * heights of each size class's dominant pass and, for lattices whose descriptor
+ * leaves the choice open (sweeps 0), for one and for two steps per pass; one
+ * class (kind and tile shape) at a time; keeps the fastest, restores the states
+ * and reports the tuned time per step.  tsmi_batch_info: out[0..2] W, H, pitch
+ * of lattice i's tile, out[3..5] strips, segments and rows per segment of its
+ * dominant pass, out[6] steps per pass (1 or 2).
  * set_state / set_exp_potential / get_state take W*H FP64 host arrays of
  * lattice i; get_state syncs. */
 typedef struct tsmi_batch tsmi_batch;
@@ -167,7 +170,7 @@ int tsmi_batch_set_exp_potential(tsmi_batch *batch, int32_t i, const double *e_r
 int tsmi_batch_evolve(tsmi_batch *batch, int32_t n_steps);
 int tsmi_batch_get_state(tsmi_batch *batch, int32_t i, double *re, double *im);
 int tsmi_batch_tune(tsmi_batch *batch, int32_t steps, double *ms_per_step);
-int tsmi_batch_info(const tsmi_batch *batch, int32_t i, int32_t out[6]);
+int tsmi_batch_info(const tsmi_batch *batch, int32_t i, int32_t out[7]);
 int tsmi_batch_synchronize(tsmi_batch *batch);
 
 /* Two-component systems: Hamiltonian2Component (src/trottersuzuki.h:459-497)

@@ -86,14 +86,17 @@ def test_batch_fp32_matches_single_context(oracle):
 
 
 def test_batch_tune_keeps_states(oracle):
+    """The autotuner (segment heights and, where free, one or two steps per pass)
+    leaves every state where it was: the evolution continues exactly."""
     from tsmi.batch import HipBatch
     wls = workloads(MIXED)
     with HipBatch(wls, precision=64, e_planes=oracle_e(oracle, wls)) as b:
         b.evolve(3)
         ms = b.tune(steps=2)
         assert ms > 0
-        rows = [b.info(i)["rows_per_segment"] for i in range(len(wls))]
-        assert all(r >= 2 and r % 2 == 0 for r in rows)
+        info = [b.info(i) for i in range(len(wls))]
+        assert all(r["rows_per_segment"] >= 2 and r["rows_per_segment"] % 2 == 0 for r in info)
+        assert all(r["steps_per_pass"] in (1, 2) for r in info)
         b.evolve(4)
         for i, wl in enumerate(wls):
             assert rel_l2(b.state(i), oracle_ref(oracle, wl, [3, 4])) < 1e-12, i

@@ -61,8 +61,10 @@ struct Lattice {
     int ix0 = 0, ix1 = 0, iy0 = 0, iy1 = 0;
     bool fixed_rows = false;
     // two steps per pass: the fused decomposition over the Wd x Hd allocation
-    // (the reference tile plus 4 more halo cells per periodic side when halo8)
-    bool fused = false, halo8 = false;
+    // (the reference tile plus 4 more halo cells per periodic side when halo8).
+    // fusable: the layout allows both passes; fused: the pass in use; tunable:
+    // the autotuner may switch between them (descriptor sweeps 0)
+    bool fusable = false, fused = false, tunable = false, halo8 = false;
     int Wd = 0, Hd = 0;
     int n_strips2 = 0, seg_rows2 = 0, n_segs2 = 0, n_waves2 = 0;
     StepCoefs k;
@@ -139,7 +141,7 @@ void set_rows(Lattice &L, int seg) {
 }
 
 void set_rows2(Lattice &L, int seg) {
-    if (!L.fused) return;
+    if (!L.fusable) return;
     seg = std::max(2, std::min(seg, std::min(L.Hd, max_seg_rows(L.pitch))));
     seg = (seg + 1) / 2 * 2;
     L.seg_rows2 = std::min(seg, L.Hd + (L.Hd & 1));
@@ -270,7 +272,7 @@ int upload_exchange(tsmi_batch *b) {
             for (const Lattice &L : b->lat) {
                 if (L.group() != g || (!L.d.periodic_x && !L.d.periodic_y)) continue;
                 ExchParams<T> q;
-                if (L.halo8) {
+                if (L.halo8 && L.fused) {
                     q.re = (T *)(b->arena + L.psi[buf][0] - L.shift);
                     q.im = (T *)(b->arena + L.psi[buf][1] - L.shift);
                     q.W = L.Wd; q.H = L.Hd; q.hx = 2 * L.d.halo_x; q.hy = 2 * L.d.halo_y;
@@ -286,6 +288,11 @@ int upload_exchange(tsmi_batch *b) {
             }
             b->n_exch[g] = (int)h.size();
             b->exch_chunks[g] = (int)std::min<long>(kExchChunks, std::max<long>(1, (most + 1023) / 1024));
+            if (b->exch[g][buf]) {
+                TSMI_HIPB(hipStreamSynchronize(b->stream));
+                TSMI_HIPB(hipFree(b->exch[g][buf]));
+                b->exch[g][buf] = nullptr;
+            }
             if (h.empty()) continue;
             TSMI_HIPB(hipMalloc(&b->exch[g][buf], h.size() * sizeof(ExchParams<T>)));
             TSMI_HIPB(hipMemcpyAsync(b->exch[g][buf], h.data(), h.size() * sizeof(ExchParams<T>),
@@ -537,9 +544,12 @@ int tsmi_batch_create(const tsmi_desc *descs, int32_t n, tsmi_batch **out) {
         L.d = d;
         L.W = d.dim_x + (d.periodic_x ? 2 * d.halo_x : 0);
         L.H = d.dim_y + (d.periodic_y ? 2 * d.halo_y : 0);
-        // 0 chooses two steps per pass wherever they are exact
-        L.fused = d.sweeps != 1 && !d.imag_time && (d.periodic_x != 0) == (d.periodic_y != 0);
-        L.halo8 = L.fused && d.periodic_x;
+        // 0 chooses two steps per pass wherever they are exact (the autotuner may
+        // go back to one); 2 insists on them
+        L.fusable = d.sweeps != 1 && !d.imag_time && (d.periodic_x != 0) == (d.periodic_y != 0);
+        L.fused = L.fusable;
+        L.tunable = L.fusable && d.sweeps == 0;
+        L.halo8 = L.fusable && d.periodic_x;
         L.Wd = L.W + (L.halo8 ? 2 * d.halo_x : 0);
         L.Hd = L.H + (L.halo8 ? 2 * d.halo_y : 0);
         L.pitch = (L.Wd + 63) / 64 * 64;
@@ -547,7 +557,7 @@ int tsmi_batch_create(const tsmi_desc *descs, int32_t n, tsmi_batch **out) {
         L.n_strips = (L.W + sw - 1) / sw;
         L.fixed_rows = d.tile_rows > 0;
         set_rows(L, d.tile_rows > 0 ? d.tile_rows : kDefaultSegRows);
-        if (L.fused) {
+        if (L.fusable) {
             L.n_strips2 = (L.Wd + sw2 - 1) / sw2;
             set_rows2(L, d.tile_rows > 0 ? d.tile_rows : kDefaultSegRows);
         }
@@ -732,8 +742,32 @@ int tsmi_batch_get_state(tsmi_batch *b, int32_t i, double *re, double *im) {
 
 int tsmi_batch_tune(tsmi_batch *b, int32_t steps, double *ms_per_step) {
     if (!b || steps < 1) return set_error(TSMI_E_INVALID, "tsmi_batch_tune: need a batch and steps >= 1");
-    // a trial reaches the fused passes only from 3 steps on
-    const int tsteps = b->n_items[2] > 0 ? std::max(steps, 6) : steps;
+    bool any_fusable = false, any_tunable = false;
+    for (const Lattice &L : b->lat) {
+        any_fusable = any_fusable || L.fusable;
+        any_tunable = any_tunable || L.tunable;
+    }
+    // A trial reaches the fused passes only from 3 steps on, and a call of n steps
+    // has 2 (or 3) single steps beside its fused passes: comparing one with two
+    // steps per pass takes trials long enough for the passes to dominate.
+    const int tsteps = any_tunable ? std::max(steps, 24) : (any_fusable ? std::max(steps, 6) : steps);
+    // A lattice may change group: first bring both groups' states into the same
+    // buffer, so that every membership finds its state where its group looks.
+    if (b->cur[1] != b->cur[0]) {
+        for (const Lattice &L : b->lat) {
+            if (L.group() != 1) continue;
+            const size_t pb = (size_t)L.pitch * (size_t)L.Hd * (size_t)b->esz;
+            for (int c = 0; c < 2; ++c)
+                TSMI_HIPB(hipMemcpyAsync(b->arena + L.psi[b->cur[0]][c] - L.shift, b->arena + L.psi[b->cur[1]][c] - L.shift,
+                                         pb, hipMemcpyDeviceToDevice, b->stream));
+        }
+        b->cur[1] = b->cur[0];
+        drop_graphs(b);
+    }
+    auto relayout = [&]() -> int {
+        const int st = b->esz == 4 ? upload_exchange<float>(b) : upload_exchange<double>(b);
+        return st ? st : upload_layout(b);
+    };
     // the trials advance the states: keep a copy of every buffer and restore it
     char *backup = nullptr;
     TSMI_HIPB(hipMalloc((void **)&backup, b->arena_bytes));
@@ -752,7 +786,7 @@ int tsmi_batch_tune(tsmi_batch *b, int32_t steps, double *ms_per_step) {
     }
     int cur[2] = {b->cur[0], b->cur[1]};
     auto trial = [&](float *ms) -> int {
-        if (int st = upload_layout(b)) return st;
+        if (int st = relayout()) return st;
         if (int st = batch_run(b, 1, cur)) return st;   // warm-up
         TSMI_HIPB(hipEventRecord(t0, b->stream));
         if (int st = batch_run(b, tsteps, cur)) return st;
@@ -761,11 +795,12 @@ int tsmi_batch_tune(tsmi_batch *b, int32_t steps, double *ms_per_step) {
         TSMI_HIPB(hipEventElapsedTime(ms, t0, t1));
         return TSMI_OK;
     };
-    // size classes: lattices of the same group and tile shape, largest first
+    // size classes: lattices of the same kind (one step per pass, two, or
+    // either) and tile shape, largest first
     std::map<std::tuple<int, int, int>, std::vector<int>> classes;
     for (int i = 0; i < b->n; ++i) {
         const Lattice &L = b->lat[i];
-        if (!L.fixed_rows) classes[std::make_tuple(L.group(), L.Hd, L.Wd)].push_back(i);
+        if (!L.fixed_rows || L.tunable) classes[std::make_tuple(L.tunable ? 2 : L.group(), L.Hd, L.Wd)].push_back(i);
     }
     std::vector<std::tuple<int, int, int>> order;
     for (auto &c : classes) order.push_back(c.first);
@@ -777,31 +812,50 @@ int tsmi_batch_tune(tsmi_batch *b, int32_t steps, double *ms_per_step) {
     float best_all = 0.f;
     for (const auto &key : order) {
         const std::vector<int> &members = classes[key];
-        // the dominant pass of the class: fused passes for group 1
-        const bool two = std::get<0>(key) == 1;
+        const int kind = std::get<0>(key);
         const int height = std::get<1>(key);
-        auto apply = [&](int rows) {
+        const Lattice &L0 = b->lat[members[0]];
+        // the pass kinds to try: the class's own, or both for a tunable class;
+        // the rows of the class's dominant pass, unless fixed by the descriptor
+        std::vector<int> kinds = kind == 2 ? std::vector<int>{1, 0} : std::vector<int>{kind};
+        auto apply = [&](int two, int rows) {
             for (int i : members) {
-                if (two) set_rows2(b->lat[i], rows);
-                else set_rows(b->lat[i], rows);
+                Lattice &L = b->lat[i];
+                L.fused = two == 1;
+                if (L.fixed_rows) continue;
+                if (two) set_rows2(L, rows);
+                else set_rows(L, rows);
             }
         };
-        const int keep = two ? b->lat[members[0]].seg_rows2 : b->lat[members[0]].seg_rows;
-        int best_rows = keep;
+        // A trial times the whole batch, so a small class moves it little against
+        // the timing noise: the class's current setting is timed first and a
+        // candidate replaces the best only when it is faster by more than 1%.
+        int best_kind = L0.fused ? 1 : 0;
+        int best_rows = L0.fused ? L0.seg_rows2 : L0.seg_rows;
         float best = -1.f;
-        for (int rows : cand) {
-            if (rows > height + 1 && rows != cand[0]) break;
-            apply(rows);
-            float ms = 0.f;
-            st = trial(&ms);
+        apply(best_kind, best_rows);
+        st = trial(&best);
+        if (st) break;
+        const int kind0 = best_kind, rows0 = best_rows;
+        for (int two : kinds) {
+            if (L0.fixed_rows && two == kind0) continue;   // already timed
+            for (int rows : cand) {
+                if (rows > height + 1 && rows != cand[0]) break;
+                if (two == kind0 && rows == rows0) continue;
+                apply(two, rows);
+                float ms = 0.f;
+                st = trial(&ms);
+                if (st) break;
+                if (ms < 0.99f * best) { best = ms; best_kind = two; best_rows = rows; }
+                if (L0.fixed_rows) break;   // one trial per kind
+            }
             if (st) break;
-            if (best < 0.f || ms < best) { best = ms; best_rows = rows; }
         }
         if (st) break;
-        apply(best_rows);
+        apply(best_kind, best_rows);
         best_all = best;
     }
-    if (!st) st = upload_layout(b);
+    if (!st) st = relayout();
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     restore();
@@ -810,12 +864,13 @@ int tsmi_batch_tune(tsmi_batch *b, int32_t steps, double *ms_per_step) {
     return TSMI_OK;
 }
 
-int tsmi_batch_info(const tsmi_batch *b, int32_t i, int32_t out[6]) {
+int tsmi_batch_info(const tsmi_batch *b, int32_t i, int32_t out[7]) {
     if (!b || !out || i < 0 || i >= b->n) return set_error(TSMI_E_INVALID, "tsmi_batch_info: no such lattice");
     const Lattice &L = b->lat[i];
     out[0] = L.W; out[1] = L.H; out[2] = L.pitch;
     if (L.fused) { out[3] = L.n_strips2; out[4] = L.n_segs2; out[5] = L.seg_rows2; }
     else { out[3] = L.n_strips; out[4] = L.n_segs; out[5] = L.seg_rows; }
+    out[6] = L.fused ? 2 : 1;
     return TSMI_OK;
 }
 

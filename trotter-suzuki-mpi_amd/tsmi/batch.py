@@ -96,6 +96,6 @@ class HipBatch:
         return out.value
 
     def info(self, i: int) -> dict:
-        out = (C.c_int32 * 6)()
+        out = (C.c_int32 * 7)()
         _lib.check(self._L.tsmi_batch_info(self._h, i, out), "tsmi_batch_info")
-        return dict(zip(("W", "H", "pitch", "strips", "segments", "rows_per_segment"), list(out)))
+        return dict(zip(("W", "H", "pitch", "strips", "segments", "rows_per_segment", "steps_per_pass"), list(out)))
