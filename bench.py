@@ -158,7 +158,9 @@ def precision_study(steps: int = 10000):
     e64, e32 = out[64]["energy"], out[32]["energy"]
     iy, ix = lat.inner_y, lat.inner_x
     a32, a64 = out[32]["psi"][iy, ix], out[64]["psi"][iy, ix]
-    rel = float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64))
+    # plain pairwise sums, not BLAS dot: under rocprofv3's preloaded library the
+    # BLAS path returned NaN for these finite (partly subnormal) arrays
+    rel = float(np.sqrt(np.sum(np.abs(a32 - a64) ** 2) / np.sum(np.abs(a64) ** 2)))
     sites = lat.dim_x * lat.dim_y
     return {"workload": f"imaginary-time GPE ground state, {lat.dim_x}x{lat.dim_y} closed, side 24, g=100, "
                         f"dt=1e-4, {steps} steps",

@@ -42,9 +42,11 @@ struct tsmi_ctx {
     int sweeps = 1;
     int n_strips2 = 0, n_segs2 = 0, seg_rows2 = 0, n_waves2 = 0;
     double *v = nullptr, *xc = nullptr, *yc = nullptr;
-    double *div = nullptr;            // two divisor slots (imaginary time)
-    double *partials = nullptr;       // one per wave
-    unsigned *counter = nullptr;
+    // imaginary time: two slots of per-wave partial sums (one per step parity),
+    // their counts, and one divisor for host-side readers (divisor_kernel)
+    double *div = nullptr;
+    double *partials = nullptr;
+    int *part_n = nullptr;
     double *obs_partials = nullptr;
     int obs_blocks = 0;
     int cur = 0, div_slot = 0;

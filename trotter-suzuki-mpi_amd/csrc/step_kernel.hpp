@@ -59,14 +59,48 @@ struct StepParams {
     // pass is multiplied by un (= 1/g of that axis).
     T mH, tH, mV, tV, unH, unV;
     T c_nl, c_lhy;               // nonlinear couplings, already rescaled to the frame
-    // imaginary time only
-    const double *div_in;        // divisor of the previous renormalisation (nullptr: 1)
-    double *div_out;             // divisor computed from this step's output
-    double *partials;            // one FP64 partial sum per wave
-    unsigned *counter;           // zero between launches
+    // Imaginary time only.  The renormalisation of a step's output is applied
+    // as the next step loads it: every wave of a step stores its FP64 partial
+    // sum of |psi|^2, wave 0 stores their count, and every wave of the next
+    // launch sums them in wave order (renorm_divisor).  The launch boundary makes
+    // the stores visible; no wave waits for another inside a launch.
+    const double *part_in;       // partial sums of the step that wrote the input (nullptr: divisor 1)
+    const int *n_in;             // their count (0: divisor 1)
+    double *part_out;            // this step's partial sums, one per wave
+    int *n_out;                  // wave 0 stores n_waves here
     double norm_target, dxdy;
     int ix0, ix1, iy0, iy1;      // inner region (tile coordinates)
 };
+
+// The imaginary-time divisor sqrt(N / N_target), N = dxdy * (sum of a step's
+// per-wave partials), or 1 for no partials or no target (src/cpukernel.cpp:
+// 548-560 divides by it).  Lane-strided Neumaier sums in wave order, then an
+// xor butterfly: partner lanes add the same two operands, so every lane of the
+// wave returns the same value, and every caller the same bits.
+__device__ __forceinline__ double renorm_divisor(const double *part, int n, double dxdy, double target, int lane) {
+    double a = 0.0, c = 0.0;
+    for (int i = lane; i < n; i += kWave) {
+        const double v = part[i];
+        const double t = a + v;
+        c += (fabs(a) >= fabs(v)) ? (a - t) + v : (v - t) + a;
+        a = t;
+    }
+    double tot = a + c;
+    for (int o = kWave / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, kWave);
+    if (n <= 0 || target == 0.0) return 1.0;
+    const double nrm = tot * dxdy;
+    return sqrt(nrm / target);
+}
+
+// One wave: the divisor of a pending renormalisation, for host-side readers
+// (a template only so that every source including this header shares it).
+template <typename = void>
+__global__ void __launch_bounds__(64) divisor_kernel(const double *part, const int *n, double dxdy, double target,
+                                                     double *out) {
+    const int lane = threadIdx.x;
+    const double d = renorm_divisor(part, __builtin_amdgcn_readfirstlane(*n), dxdy, target, lane);
+    if (lane == 0) *out = d;
+}
 
 // ---------------------------------------------------------------------------
 // pair updates
@@ -543,7 +577,9 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     const bool interior = MODE >= 0 && X0 >= 0 && X0 + kWave * VEC <= p.W;
 
     double divd = 1.0;
-    if (IMAG && p.div_in) divd = *p.div_in;
+    if (IMAG && p.part_in)
+        divd = renorm_divisor(p.part_in, __builtin_amdgcn_readfirstlane(*p.n_in), p.dxdy, p.norm_target, lane);
+    const bool scale_in = IMAG && p.part_in && divd != 1.0;
 
     double acc = 0.0, comp = 0.0;   // Neumaier sum of |psi|^2 over stored inner cells
 
@@ -572,7 +608,7 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
         buf_load<T, VEC>(r_in_re, v_ld, so, r.re);
         buf_load<T, VEC>(r_in_im, v_ld, so, r.im);
-        if (IMAG && p.div_in) {
+        if (scale_in) {
 #pragma unroll
             for (int k = 0; k < VEC; ++k) {
                 r.re[k] = T(double(r.re[k]) / divd);
@@ -728,34 +764,12 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     }
 
     if (IMAG) {
-        // deterministic renormalisation: one partial per wave; the wave that
-        // finishes last sums them in wave order and publishes the divisor
+        // this wave's partial sum for the next step's divisor
         double s = acc + comp;
         for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
-        unsigned ticket = 0u;
         if (lane == 0) {
-            p.partials[wave] = s;
-            __threadfence();
-            ticket = atomicAdd(p.counter, 1u);
-        }
-        // lane 0's ticket, as a wave-uniform value: the branch below is uniform
-        ticket = __builtin_amdgcn_readfirstlane(ticket);
-        if (ticket == (unsigned)p.n_waves - 1) {
-            __threadfence();
-            double a2 = 0.0, c2 = 0.0;
-            for (int i = lane; i < p.n_waves; i += kWave) {
-                const double v = __hip_atomic_load(p.partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const double t2 = a2 + v;
-                c2 += (fabs(a2) >= fabs(v)) ? (a2 - t2) + v : (v - t2) + a2;
-                a2 = t2;
-            }
-            double tot = a2 + c2;
-            for (int o = kWave / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, kWave);
-            if (lane == 0) {
-                const double nrm = tot * p.dxdy;
-                *p.div_out = (p.norm_target != 0.0) ? sqrt(nrm / p.norm_target) : 1.0;
-                __hip_atomic_store(p.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
+            p.part_out[wave] = s;
+            if (wave == 0) *p.n_out = p.n_waves;
         }
     }
 }

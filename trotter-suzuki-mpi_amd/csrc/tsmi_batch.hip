@@ -74,7 +74,9 @@ struct Lattice {
     size_t e8[2] = {0, 0};                    // allocation of the E the fused passes read
     size_t ph = 0, ph8 = 0;                   // the one-plane form of E: view, and what the fused passes read
     bool ph_ok = false;                       // E has that form (|E| = 1, or E > 0)
-    size_t partials = 0, div = 0, counter = 0;
+    // imaginary time: partial sums of |psi|^2 per buffer parity (max_waves
+    // each), their two counts, and a divisor for host-side readers
+    size_t partials = 0, div = 0, part_n = 0;
     int group() const { return fused ? 1 : 0; }
 };
 
@@ -174,11 +176,14 @@ StepParams<T> params_of(const tsmi_batch *b, const Lattice &L, int parity) {
     set_coefs(p, L);
     p.ix0 = L.ix0; p.ix1 = L.ix1; p.iy0 = L.iy0; p.iy1 = L.iy1;
     if (b->imag) {
-        // the divisor slots follow the buffer parity; a fresh state holds 1.0
-        p.div_in = (const double *)(a + L.div) + parity;
-        p.div_out = (double *)(a + L.div) + (1 - parity);
-        p.partials = (double *)(a + L.partials);
-        p.counter = (unsigned *)(a + L.counter);
+        // the partial-sum slots follow the buffer parity; a fresh state has a
+        // count of 0 (divisor 1), and the counts travel with the sums, so a
+        // pending renormalisation survives a change of segment heights
+        const size_t mw = (size_t)L.max_waves;
+        p.part_in = (const double *)(a + L.partials) + (size_t)parity * mw;
+        p.n_in = (const int *)(a + L.part_n) + parity;
+        p.part_out = (double *)(a + L.partials) + (size_t)(1 - parity) * mw;
+        p.n_out = (int *)(a + L.part_n) + (1 - parity);
         p.norm_target = L.d.norm_target;
         p.dxdy = L.d.delta_x * L.d.delta_y;
     }
@@ -471,8 +476,8 @@ int upload_host_plane(tsmi_batch *b, const Lattice &L, size_t off, const double 
 
 int reset_divisor(tsmi_batch *b, const Lattice &L) {
     if (!b->imag) return TSMI_OK;
-    const double one[2] = {1.0, 1.0};
-    TSMI_HIPB(hipMemcpyAsync(b->arena + L.div, one, sizeof(one), hipMemcpyHostToDevice, b->stream));
+    const int none[2] = {0, 0};
+    TSMI_HIPB(hipMemcpyAsync(b->arena + L.part_n, none, sizeof(none), hipMemcpyHostToDevice, b->stream));
     TSMI_HIPB(hipStreamSynchronize(b->stream));
     return TSMI_OK;
 }
@@ -577,9 +582,9 @@ int tsmi_batch_create(const tsmi_desc *descs, int32_t n, tsmi_batch **out) {
         L.ph = off + L.shift; off += pb;
         if (L.halo8) { L.ph8 = off; off += pb; }
         else L.ph8 = L.ph;
-        L.partials = off; off += align256((size_t)L.max_waves * sizeof(double));
+        L.partials = off; off += align256(2 * (size_t)L.max_waves * sizeof(double));
         L.div = off; off += 256;
-        L.counter = off; off += 256;
+        L.part_n = off; off += 256;
     }
     b->mode = mode;
     b->arena_bytes = off;
@@ -726,9 +731,14 @@ int tsmi_batch_get_state(tsmi_batch *b, int32_t i, double *re, double *im) {
                                (size_t)L->W * es, L->H, hipMemcpyDeviceToHost, b->stream));
     TSMI_HIPB(hipMemcpy2DAsync(ti.data(), (size_t)L->W * es, b->arena + L->psi[buf][1], (size_t)L->pitch * es,
                                (size_t)L->W * es, L->H, hipMemcpyDeviceToHost, b->stream));
-    if (b->imag)
-        TSMI_HIPB(hipMemcpyAsync(&div, (const double *)(b->arena + L->div) + buf, sizeof(double),
-                                 hipMemcpyDeviceToHost, b->stream));
+    if (b->imag) {
+        hipLaunchKernelGGL(divisor_kernel<>, dim3(1), dim3(kWave), 0, b->stream,
+                           (const double *)(b->arena + L->partials) + (size_t)buf * (size_t)L->max_waves,
+                           (const int *)(b->arena + L->part_n) + buf, L->d.delta_x * L->d.delta_y, L->d.norm_target,
+                           (double *)(b->arena + L->div));
+        TSMI_HIPB(hipGetLastError());
+        TSMI_HIPB(hipMemcpyAsync(&div, b->arena + L->div, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    }
     TSMI_HIPB(hipStreamSynchronize(b->stream));
     for (size_t k = 0; k < n; ++k) {
         double vr, vi;

@@ -121,7 +121,7 @@ int free_all(tsmi_ctx *c) {
         for (int k = 0; k < 2; ++k)
             if (c->psi_base[b][k]) (void)hipFree(c->psi_base[b][k]);
     void *ptrs[] = {c->e_base[0], c->e_base[1], c->e8[0], c->e8[1], c->ph_base, c->ph8, c->flag, c->v_base, c->xc,
-                    c->yc, c->div, c->partials, c->counter, c->obs_partials, c->ck, c->adj, c->grad};
+                    c->yc, c->div, c->partials, c->part_n, c->obs_partials, c->ck, c->adj, c->grad};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &k : c->ev)
@@ -148,10 +148,21 @@ int upload_plane(tsmi_ctx *c, void *dst, const double *src) {
     return TSMI_OK;
 }
 
+// The divisor of a pending imaginary-time renormalisation, computed into c->div
+// on the device from the last step's partial sums (nullptr: none pending).
+const double *pending_div(tsmi_ctx *c) {
+    if (!c->d.imag_time || !c->div_pending) return nullptr;
+    hipLaunchKernelGGL(divisor_kernel<>, dim3(1), dim3(kWave), 0, c->stream,
+                       (const double *)(c->partials + (size_t)c->div_slot * c->n_waves),
+                       (const int *)(c->part_n + c->div_slot), c->d.delta_x * c->d.delta_y, c->d.norm_target, c->div);
+    return c->div;
+}
+
 int current_div(tsmi_ctx *c, double *out) {
     *out = 1.0;
-    if (c->d.imag_time && c->div_pending) {
-        TSMI_HIP(hipMemcpyAsync(out, c->div + c->div_slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (const double *d = pending_div(c)) {
+        TSMI_HIP(hipGetLastError());
+        TSMI_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         TSMI_HIP(hipStreamSynchronize(c->stream));
     }
     return TSMI_OK;
@@ -376,10 +387,11 @@ int launch_step(tsmi_ctx *c, int exchange) {
     p.c_nl = (T)c->c_nl_s; p.c_lhy = (T)c->c_lhy_s;
     p.ix0 = c->ix0; p.ix1 = c->ix1; p.iy0 = c->iy0; p.iy1 = c->iy1;
     if (IMAG) {
-        p.div_in = c->div_pending ? c->div + c->div_slot : nullptr;
-        p.div_out = c->div + (1 - c->div_slot);
-        p.partials = c->partials;
-        p.counter = c->counter;
+        const size_t nw = (size_t)c->n_waves;
+        p.part_in = c->div_pending ? c->partials + (size_t)c->div_slot * nw : nullptr;
+        p.n_in = c->part_n + c->div_slot;
+        p.part_out = c->partials + (size_t)(1 - c->div_slot) * nw;
+        p.n_out = c->part_n + (1 - c->div_slot);
         p.norm_target = c->d.norm_target;
         p.dxdy = c->d.delta_x * c->d.delta_y;
     }
@@ -733,8 +745,8 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
         {&c->ph_base, pb}, {&c->ph8, halo8 ? pb : 0}, {(void **)&c->flag, 64},
         {(void **)&c->v_base, pd},
         {(void **)&c->xc, (size_t)c->W * sizeof(double)}, {(void **)&c->yc, (size_t)c->H * sizeof(double)},
-        {(void **)&c->div, 2 * sizeof(double)}, {(void **)&c->partials, (size_t)c->n_waves * sizeof(double)},
-        {(void **)&c->counter, 64}, {(void **)&c->obs_partials, (size_t)c->obs_blocks * OBS_COUNT * sizeof(double)},
+        {(void **)&c->div, 2 * sizeof(double)}, {(void **)&c->partials, 2 * (size_t)c->n_waves * sizeof(double)},
+        {(void **)&c->part_n, 64}, {(void **)&c->obs_partials, (size_t)c->obs_blocks * OBS_COUNT * sizeof(double)},
     };
     for (auto &a : allocs) {
         if (a.n == 0) continue;
@@ -841,7 +853,8 @@ int tsmi_set_potential_device(tsmi_ctx *c, const double *v) {
 int tsmi_get_state_device(tsmi_ctx *c, double *re, double *im) {
     if (!c || !re || !im) return fail(TSMI_E_INVALID, "tsmi_get_state_device: null argument");
     if (!c->has_state) return fail(TSMI_E_INVALID, "tsmi_get_state_device: no state");
-    const double *div = (c->d.imag_time && c->div_pending) ? c->div + c->div_slot : nullptr;
+    const double *div = pending_div(c);
+    if (div) TSMI_HIP(hipGetLastError());
     const int blocks = 2048;
     for (int k = 0; k < 2; ++k) {
         const void *src = k ? c->psi_im[c->cur] : c->psi_re[c->cur];
