@@ -160,41 +160,93 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
         const int y = t - 8;
         if (valid(y) && y >= y_out_lo && y < y_out_hi) emit(y, b);
     };
-    auto beat_fast = [&](auto tp, int t, const Row<T, VEC> &in, const Row<T, VEC> &ea, const Row<T, VEC> &eb) {
-        constexpr int TP = decltype(tp)::value;
-        Row<T, VEC> a, b;
-        pipe_beat_fast<TP, LHY, MODE, PH>(A, in, ea, a, kH, kV, c, cl, mag);
-        pipe_beat_fast<TP, LHY, MODE, PH>(B, a, eb, b, kH, kV, c, cl, mag);
-        const int y = t - 8;
-        if (y >= y_out_lo && y < y_out_hi) emit(y, b);
-    };
 
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
+    // Two beats per iteration; state row r sits in slot in[(r - y_lo) % 2].
+    auto pair_general = [&](int t) {
+        beat(I0(), t, inA, eA, fA);
+        if (t + 2 < y_hi) load_row(r_in_re, r_in_im, t + 2, inA);
+        if (t < y_hi) load_e(t, eA);
+        if (valid(t - 4)) load_e(t - 4, fA);
+        beat(I1(), t + 1, inB, eB, fB);
+        if (t + 3 < y_hi) load_row(r_in_re, r_in_im, t + 3, inB);
+        if (t + 1 < y_hi) load_e(t + 1, eB);
+        if (valid(t - 3)) load_e(t - 3, fB);
+    };
+    // The steady state of an interior strip (t >= y_lo + 8, t + 1 < y_hi).  No
+    // memory operation is conditional: a state row past the segment is
+    // addressed through the vector offset, which the buffer's range check turns
+    // into a load of zeros, and a row that leaves the second pipeline before
+    // the output range (the first four beats) gets an offset past the buffer,
+    // which drops the store.
+    auto load_row_any = [&](int y, Row<T, VEC> &r) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_load<T, VEC>(r_in_re, v_ld + so, 0u, r.re);
+        buf_load<T, VEC>(r_in_im, v_ld + so, 0u, r.im);
+    };
+    auto emit_any = [&](int y, const Row<T, VEC> &o) {
+        const unsigned off = y >= y_out_lo ? v_st + (unsigned)(y - y_lo) * row_bytes : kOffNone;
+        buf_store<T, VEC>(r_out_re, off, 0u, o.re);
+        buf_store<T, VEC>(r_out_im, off, 0u, o.im);
+    };
+    auto pair_fast = [&](int t) {
+        Row<T, VEC> a, b;
+        pipe_beat_fast<0, LHY, MODE, PH>(A, inA, eA, a, kH, kV, c, cl, mag);
+        pipe_beat_fast<0, LHY, MODE, PH>(B, a, fA, b, kH, kV, c, cl, mag);
+        emit_any(t - 8, b);
+        load_row_any(t + 2, inA);
+        load_e(t, eA);
+        load_e(t - 4, fA);
+        pipe_beat_fast<1, LHY, MODE, PH>(A, inB, eB, a, kH, kV, c, cl, mag);
+        pipe_beat_fast<1, LHY, MODE, PH>(B, a, fB, b, kH, kV, c, cl, mag);
+        emit_any(t - 7, b);
+        load_row_any(t + 3, inB);
+        load_e(t + 1, eB);
+        load_e(t - 3, fB);
+    };
     int t = y_lo;
     if (t < y_hi) load_row(r_in_re, r_in_im, t, inA);
     if (t + 1 < y_hi) load_row(r_in_re, r_in_im, t + 1, inB);
     const int t_end = y_hi + 8;
-    for (; t < t_end; t += 2) {
-        if (interior && t >= y_lo + 8 && t + 1 < y_hi) {
-            beat_fast(I0(), t, inA, eA, fA);
-            if (t + 2 < y_hi) load_row(r_in_re, r_in_im, t + 2, inA);
-            load_e(t, eA);
-            load_e(t - 4, fA);
-            beat_fast(I1(), t + 1, inB, eB, fB);
-            if (t + 3 < y_hi) load_row(r_in_re, r_in_im, t + 3, inB);
-            load_e(t + 1, eB);
-            load_e(t - 3, fB);
-        } else {
-            beat(I0(), t, inA, eA, fA);
-            if (t + 2 < y_hi) load_row(r_in_re, r_in_im, t + 2, inA);
-            if (t < y_hi) load_e(t, eA);
-            if (valid(t - 4)) load_e(t - 4, fA);
-            beat(I1(), t + 1, inB, eB, fB);
-            if (t + 3 < y_hi) load_row(r_in_re, r_in_im, t + 3, inB);
-            if (t + 1 < y_hi) load_e(t + 1, eB);
-            if (valid(t - 3)) load_e(t - 3, fB);
+    if constexpr (sizeof(T) == 4) {
+        // FP32 (measured fastest at 8192^2, 4 waves per SIMD): one loop that
+        // picks the fast or the general pair of beats per iteration.
+        for (; t < t_end; t += 2) {
+            if (interior && t >= y_lo + 8 && t + 1 < y_hi) {
+                Row<T, VEC> a, b;
+                pipe_beat_fast<0, LHY, MODE, PH>(A, inA, eA, a, kH, kV, c, cl, mag);
+                pipe_beat_fast<0, LHY, MODE, PH>(B, a, fA, b, kH, kV, c, cl, mag);
+                if (t - 8 >= y_out_lo) emit(t - 8, b);
+                if (t + 2 < y_hi) load_row(r_in_re, r_in_im, t + 2, inA);
+                load_e(t, eA);
+                load_e(t - 4, fA);
+                pipe_beat_fast<1, LHY, MODE, PH>(A, inB, eB, a, kH, kV, c, cl, mag);
+                pipe_beat_fast<1, LHY, MODE, PH>(B, a, fB, b, kH, kV, c, cl, mag);
+                if (t - 7 >= y_out_lo) emit(t - 7, b);
+                if (t + 3 < y_hi) load_row(r_in_re, r_in_im, t + 3, inB);
+                load_e(t + 1, eB);
+                load_e(t - 3, fB);
+            } else {
+                pair_general(t);
+            }
         }
+    } else {
+        // FP64: the steady state in a loop of its own, whose simpler control
+        // flow lets the compiler fit the pass in 215 VGPRs (2 waves per SIMD;
+        // the single loop needed 256 plus spills to AGPRs, 1 wave per SIMD).
+        // The iteration ends by waiting for its loads: the prefetched rows are
+        // copied between registers at the back edge, which waits for them
+        // anyway, and the early wait keeps the compiler from placing partial
+        // waits inside the next beats (measured faster).
+        if (interior) {
+            for (; t < t_end && t < y_lo + 8; t += 2) pair_general(t);   // the pipelines fill
+            for (; t + 1 < y_hi; t += 2) {
+                pair_fast(t);
+                __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+            }
+        }
+        for (; t < t_end; t += 2) pair_general(t);
     }
 }
 

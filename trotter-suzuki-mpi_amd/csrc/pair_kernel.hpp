@@ -166,7 +166,10 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
     const T cab = p.c_ab;
 
     const bool div0_on = IMAG && p.div_in[0] != nullptr, div1_on = IMAG && p.div_in[1] != nullptr;
+    // pending divisors, applied to each row as it enters the pipelines (not as
+    // it is loaded, so that the loads of the rows ahead stay in flight)
     const double div0 = div0_on ? *p.div_in[0] : 1.0, div1 = div1_on ? *p.div_in[1] : 1.0;
+    const double inv0 = 1.0 / div0, inv1 = 1.0 / div1;
     double acc0 = 0.0, cmp0 = 0.0, acc1 = 0.0, cmp1 = 0.0;
 
     const long seg0 = (long)y_lo * p.pitch;
@@ -201,20 +204,6 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
         buf_load<T, VEC>(ri0i, v_ld, so, a.im);
         buf_load<T, VEC>(ri1r, v_ld, so, b.re);
         buf_load<T, VEC>(ri1i, v_ld, so, b.im);
-        if (div0_on) {
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) {
-                a.re[k] = T(double(a.re[k]) / div0);
-                a.im[k] = T(double(a.im[k]) / div0);
-            }
-        }
-        if (div1_on) {
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) {
-                b.re[k] = T(double(b.re[k]) / div1);
-                b.im[k] = T(double(b.im[k]) / div1);
-            }
-        }
     };
     auto load_e = [&](int y, Row<T, VEC> &a, Row<T, VEC> &b) {
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
@@ -279,6 +268,8 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
                     const Row<T, VEC> &eb, T (&na)[VEC], T (&nb)[VEC]) {
         constexpr int TP = decltype(tp)::value;
         const bool v0 = valid(t), v1 = valid(t - 1), v2 = valid(t - 2), v3 = valid(t - 3), v4 = valid(t - 4);
+        if (div0_on && v0) scale_row<T, VEC>(ia, div0, inv0);
+        if (div1_on && v0) scale_row<T, VEC>(ib, div1, inv1);
         T ta[VEC], tb[VEC];
         density(ia, ta);
         density(ib, tb);

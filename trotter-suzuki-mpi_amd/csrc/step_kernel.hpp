@@ -76,20 +76,49 @@ struct StepParams {
 // per-wave partials), or 1 for no partials or no target (src/cpukernel.cpp:
 // 548-560 divides by it).  Lane-strided Neumaier sums in wave order, then an
 // xor butterfly: partner lanes add the same two operands, so every lane of the
-// wave returns the same value, and every caller the same bits.
+// wave returns the same value, and every caller the same bits.  The partials
+// are read sixteen per lane at a time (all loads issued before the first wait);
+// a lane past the end adds zeros, which leave its sum and compensation as they are.
 __device__ __forceinline__ double renorm_divisor(const double *part, int n, double dxdy, double target, int lane) {
+    constexpr int kBatch = 16;
     double a = 0.0, c = 0.0;
-    for (int i = lane; i < n; i += kWave) {
-        const double v = part[i];
-        const double t = a + v;
-        c += (fabs(a) >= fabs(v)) ? (a - t) + v : (v - t) + a;
-        a = t;
+    for (int i0 = 0; i0 < n; i0 += kBatch * kWave) {
+        double x[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) x[j] = part[min(i0 + j * kWave + lane, n - 1)];
+        __builtin_amdgcn_sched_barrier(0);   // every load of the batch before the first use
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const double v = i0 + j * kWave + lane < n ? x[j] : 0.0;
+            const double t = a + v;
+            c += (fabs(a) >= fabs(v)) ? (a - t) + v : (v - t) + a;
+            a = t;
+        }
     }
     double tot = a + c;
     for (int o = kWave / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, kWave);
     if (n <= 0 || target == 0.0) return 1.0;
     const double nrm = tot * dxdy;
     return sqrt(nrm / target);
+}
+
+// x / d for a divisor d with inv = 1 / d (both FP64): the product and one
+// fused correction (Markstein), which gives the correctly rounded quotient at
+// the cost of three multiply-adds instead of a full division.
+__device__ __forceinline__ double div_by(double x, double d, double inv) {
+    const double q = x * inv;
+    return fma(fma(-q, d, x), inv, q);
+}
+
+// Divides a row by the pending renormalisation divisor: the FP64 quotient,
+// rounded once to the row's precision.
+template <typename T, int VEC>
+__device__ __forceinline__ void scale_row(Row<T, VEC> &r, double d, double inv) {
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        r.re[k] = T(div_by(double(r.re[k]), d, inv));
+        r.im[k] = T(div_by(double(r.im[k]), d, inv));
+    }
 }
 
 // One wave: the divisor of a pending renormalisation, for host-side readers
@@ -576,10 +605,11 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     // axes use different scaled forms, so only the general form runs)
     const bool interior = MODE >= 0 && X0 >= 0 && X0 + kWave * VEC <= p.W;
 
-    double divd = 1.0;
-    if (IMAG && p.part_in)
-        divd = renorm_divisor(p.part_in, __builtin_amdgcn_readfirstlane(*p.n_in), p.dxdy, p.norm_target, lane);
-    const bool scale_in = IMAG && p.part_in && divd != 1.0;
+    // Imaginary time: the pending renormalisation of the input, applied to each
+    // row as it enters the pipeline (not as it is loaded, so that the loads of
+    // the rows ahead stay in flight); computed after the first loads are issued.
+    double divd = 1.0, dinv = 1.0;
+    bool scale_in = false;
 
     double acc = 0.0, comp = 0.0;   // Neumaier sum of |psi|^2 over stored inner cells
 
@@ -608,13 +638,6 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
         buf_load<T, VEC>(r_in_re, v_ld, so, r.re);
         buf_load<T, VEC>(r_in_im, v_ld, so, r.im);
-        if (scale_in) {
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) {
-                r.re[k] = T(double(r.re[k]) / divd);
-                r.im[k] = T(double(r.im[k]) / divd);
-            }
-        }
     };
     auto load_e = [&](int y, Row<T, VEC> &r) {
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
@@ -636,10 +659,8 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     auto valid = [&](int y) { return y >= y_lo && y < y_hi; };
 
     // Writes the row leaving the pipeline.
-    auto emit = [&](int y, const Row<T, VEC> &o4) {
-        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
-        buf_store<T, VEC>(r_out_re, v_st, so, o4.re);
-        buf_store<T, VEC>(r_out_im, v_st, so, o4.im);
+    // Imaginary time: adds the inner cells of an output row to the sum.
+    auto accumulate = [&](int y, const Row<T, VEC> &o4) {
         if (IMAG && y >= p.iy0 && y < p.iy1) {
 #pragma unroll
             for (int k = 0; k < VEC; ++k) {
@@ -653,6 +674,21 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
             }
         }
     };
+    auto emit = [&](int y, const Row<T, VEC> &o4) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_store<T, VEC>(r_out_re, v_st, so, o4.re);
+        buf_store<T, VEC>(r_out_im, v_st, so, o4.im);
+        accumulate(y, o4);
+    };
+    // The emit of the steady state, for rows below y_out_hi: a row before
+    // y_out_lo is dropped by a store offset past the buffer, not by a branch.
+    auto emit_any = [&](int y, const Row<T, VEC> &o4) {
+        const bool out = y >= y_out_lo;
+        const unsigned off = out ? v_st + (unsigned)(y - y_lo) * row_bytes : kOffNone;
+        buf_store<T, VEC>(r_out_re, off, 0u, o4.re);
+        buf_store<T, VEC>(r_out_im, off, 0u, o4.im);
+        if (out) accumulate(y, o4);
+    };
 
     // One pipeline beat for input row t: stage k takes row t-k+1 and emits row
     // t-k.  TP = t mod 2 fixes every pair pattern (segments start on even rows,
@@ -664,6 +700,7 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         const T a2 = (v1 && v2) ? mV : unV, b2 = (v1 && v2) ? tV : T(0);
         const T a3 = (v2 && v3) ? mV : unV, b3 = (v2 && v3) ? tV : T(0);
         const T a4 = (v3 && v4) ? mV : unV, b4 = (v3 && v4) ? tV : T(0);
+        if (IMAG && scale_in && v0) scale_row<T, VEC>(in, divd, dinv);
         // stage 1: V0, then H0 on row t-1
         vpair<(TP + 1) & 1, IMAG>(pend1, in, a1, b1);
         Row<T, VEC> o1 = pend1;
@@ -689,10 +726,12 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         if (v4 && y >= y_out_lo && y < y_out_hi) emit(y, o4);
     };
 
-    // The same beat in the fast form: interior strip, all five rows valid.
+    // The same beat in the fast form: interior strip, all five rows valid; the
+    // row it emits is below the output range's end (steady state).
     auto beat_fast = [&](auto tp, int t, Row<T, VEC> &in, const Row<T, VEC> &e) {
         constexpr int TP = decltype(tp)::value;
         constexpr int M = MODE < 0 ? 0 : MODE;
+        if (IMAG && scale_in) scale_row<T, VEC>(in, divd, dinv);
         vpair_fast<(TP + 1) & 1, IMAG, M>(pend1, in, kV);
         Row<T, VEC> o1 = pend1;
         pend1 = in;
@@ -710,8 +749,7 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         vpair_fast<TP & 1, IMAG, M>(pend4, o3, kV);
         const Row<T, VEC> o4 = pend4;
         pend4 = o3;
-        const int y = t - 4;
-        if (y >= y_out_lo && y < y_out_hi) emit(y, o4);
+        emit_any(t - 4, o4);
     };
 
     // Software pipeline, four rows ahead: state rows t..t+3 and E rows t-2..t+1
@@ -732,36 +770,64 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     if (t + 3 < y_hi) load_psi(t + 3, inD);
     if (t < y_hi) load_e(t, eA);
     if (t + 1 < y_hi) load_e(t + 1, eB);
-    for (; t < t_end; t += 4) {
-        // slots: state row r in in[(r - y_lo) % 4]; E row r in e[(r - y_lo) % 4]
-        if (interior && t >= y_lo + 4 && t + 3 < y_hi) {
-            beat_fast(I0(), t, inA, eC);
-            if (t + 4 < y_hi) load_psi(t + 4, inA);
-            load_e(t + 2, eC);
-            beat_fast(I1(), t + 1, inB, eD);
-            if (t + 5 < y_hi) load_psi(t + 5, inB);
-            load_e(t + 3, eD);
-            beat_fast(I0(), t + 2, inC, eA);
-            if (t + 6 < y_hi) load_psi(t + 6, inC);
-            if (t + 4 < y_hi) load_e(t + 4, eA);
-            beat_fast(I1(), t + 3, inD, eB);
-            if (t + 7 < y_hi) load_psi(t + 7, inD);
-            if (t + 5 < y_hi) load_e(t + 5, eB);
-        } else {
-            beat(I0(), t, inA, eC);
-            if (t + 4 < y_hi) load_psi(t + 4, inA);
-            if (t + 2 < y_hi) load_e(t + 2, eC);
-            beat(I1(), t + 1, inB, eD);
-            if (t + 5 < y_hi) load_psi(t + 5, inB);
-            if (t + 3 < y_hi) load_e(t + 3, eD);
-            beat(I0(), t + 2, inC, eA);
-            if (t + 6 < y_hi) load_psi(t + 6, inC);
-            if (t + 4 < y_hi) load_e(t + 4, eA);
-            beat(I1(), t + 3, inD, eB);
-            if (t + 7 < y_hi) load_psi(t + 7, inD);
-            if (t + 5 < y_hi) load_e(t + 5, eB);
-        }
+    if (IMAG && p.part_in) {
+        divd = renorm_divisor(p.part_in, __builtin_amdgcn_readfirstlane(*p.n_in), p.dxdy, p.norm_target, lane);
+        dinv = 1.0 / divd;
+        scale_in = divd != 1.0;
     }
+    // slots: state row r in in[(r - y_lo) % 4]; E row r in e[(r - y_lo) % 4]
+    auto group_general = [&](int t) {
+        beat(I0(), t, inA, eC);
+        if (t + 4 < y_hi) load_psi(t + 4, inA);
+        if (t + 2 < y_hi) load_e(t + 2, eC);
+        beat(I1(), t + 1, inB, eD);
+        if (t + 5 < y_hi) load_psi(t + 5, inB);
+        if (t + 3 < y_hi) load_e(t + 3, eD);
+        beat(I0(), t + 2, inC, eA);
+        if (t + 6 < y_hi) load_psi(t + 6, inC);
+        if (t + 4 < y_hi) load_e(t + 4, eA);
+        beat(I1(), t + 3, inD, eB);
+        if (t + 7 < y_hi) load_psi(t + 7, inD);
+        if (t + 5 < y_hi) load_e(t + 5, eB);
+    };
+    // The steady state of an interior strip (t >= y_lo + 4, t + 3 < y_hi).  No
+    // memory operation is conditional, so the compiler keeps the loads of the
+    // rows ahead in flight across iterations (a merge of paths with different
+    // counts of outstanding operations makes it drain them): a row past the
+    // segment is addressed through the vector offset, which the buffer's range
+    // check turns into a load of zeros (such a row only meets pairs with
+    // coefficients (1, 0)), and a row before the output range (the first four
+    // beats) gets a store offset past the buffer, which drops it.
+    auto load_psi_any = [&](int y, Row<T, VEC> &r) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_load<T, VEC>(r_in_re, v_ld + so, 0u, r.re);
+        buf_load<T, VEC>(r_in_im, v_ld + so, 0u, r.im);
+    };
+    auto load_e_any = [&](int y, Row<T, VEC> &r) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_load<T, VEC>(r_e_re, v_ld + so, 0u, r.re);
+        if (!IMAG && !PH) buf_load<T, VEC>(r_e_im, v_ld + so, 0u, r.im);
+    };
+    auto group_fast = [&](int t) {
+        beat_fast(I0(), t, inA, eC);
+        load_psi_any(t + 4, inA);
+        load_e_any(t + 2, eC);
+        beat_fast(I1(), t + 1, inB, eD);
+        load_psi_any(t + 5, inB);
+        load_e_any(t + 3, eD);
+        beat_fast(I0(), t + 2, inC, eA);
+        load_psi_any(t + 6, inC);
+        load_e_any(t + 4, eA);
+        beat_fast(I1(), t + 3, inD, eB);
+        load_psi_any(t + 7, inD);
+        load_e_any(t + 5, eB);
+    };
+    if (interior) {
+        // rows y_lo .. y_lo+3: the pipeline fills
+        for (; t < t_end && t < y_lo + 4; t += 4) group_general(t);
+        for (; t + 3 < y_hi; t += 4) group_fast(t);
+    }
+    for (; t < t_end; t += 4) group_general(t);
 
     if (IMAG) {
         // this wave's partial sum for the next step's divisor

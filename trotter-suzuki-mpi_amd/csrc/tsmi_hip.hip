@@ -93,7 +93,11 @@ constexpr int kBlock = 256;          // 4 wavefronts per workgroup
 #ifndef TSMI_FP32_VEC2
 #define TSMI_FP32_VEC2 2
 #endif
+#ifndef TSMI_FP32_VEC_IMAG
+#define TSMI_FP32_VEC_IMAG 4
+#endif
 constexpr int kVec32 = TSMI_FP32_VEC;
+constexpr int kVec32I = TSMI_FP32_VEC_IMAG;   // the imaginary-time single step
 constexpr int kVec32F = TSMI_FP32_VEC2;
 constexpr int kObsBlocksMax = 1024;
 
@@ -420,7 +424,7 @@ hipError_t occupancy_of(int *blocks) {
 
 hipError_t occupancy_blocks(int precision, bool imag, bool lhy, int *blocks) {
     if (precision == 32) {
-        if (imag) return lhy ? occupancy_of<float, true, kVec32, true>(blocks) : occupancy_of<float, true, kVec32, false>(blocks);
+        if (imag) return lhy ? occupancy_of<float, true, kVec32I, true>(blocks) : occupancy_of<float, true, kVec32I, false>(blocks);
         return lhy ? occupancy_of<float, false, kVec32, true>(blocks) : occupancy_of<float, false, kVec32, false>(blocks);
     }
     if (imag) return lhy ? occupancy_of<double, true, 2, true>(blocks) : occupancy_of<double, true, 2, false>(blocks);
@@ -444,17 +448,17 @@ int step_mode(tsmi_ctx *c, int exchange) {
                       : step_form<T, IMAG, VEC, LHY, false>(c, exchange);
 }
 
-template <typename T, int VEC>
+template <typename T, int VEC, int VEC_IMAG>
 int step_typed(tsmi_ctx *c, int exchange) {
     // the LHY term is compiled in only when g_LHY != 0 (it is exactly zero otherwise)
     const bool lhy = c->c_lhy != 0.0;
     if (c->d.imag_time)
-        return lhy ? step_mode<T, true, VEC, true>(c, exchange) : step_mode<T, true, VEC, false>(c, exchange);
+        return lhy ? step_mode<T, true, VEC_IMAG, true>(c, exchange) : step_mode<T, true, VEC_IMAG, false>(c, exchange);
     return lhy ? step_mode<T, false, VEC, true>(c, exchange) : step_mode<T, false, VEC, false>(c, exchange);
 }
 
 int step_once(tsmi_ctx *c, int exchange) {
-    return c->esz == 4 ? step_typed<float, kVec32>(c, exchange) : step_typed<double, 2>(c, exchange);
+    return c->esz == 4 ? step_typed<float, kVec32, kVec32I>(c, exchange) : step_typed<double, 2, 2>(c, exchange);
 }
 
 int exchange_planes(tsmi_ctx *c, void *re, void *im) {
@@ -662,7 +666,7 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     c->off_el = halo8 ? (size_t)d.halo_y * (size_t)pitch + (size_t)d.halo_x : 0;
     c->sweeps = sweeps;
     c->esz = d.precision == 32 ? 4 : 8;
-    c->vec = c->esz == 4 ? kVec32 : 2;
+    c->vec = c->esz == 4 ? (d.imag_time ? kVec32I : kVec32) : 2;
     const int vec2 = c->esz == 4 ? kVec32F : 2;
     const int sw = kWave * c->vec - 2 * kApron;
     c->n_strips = (c->W + sw - 1) / sw;
