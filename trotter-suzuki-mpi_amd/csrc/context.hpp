@@ -41,6 +41,7 @@ struct tsmi_ctx {
     // fused two-step passes (fused_kernel.hpp)
     int sweeps = 1;
     int n_strips2 = 0, n_segs2 = 0, seg_rows2 = 0, n_waves2 = 0;
+    int geo2_key = -1;                    // (form, E form) the segments were sized for; -1: none
     double *v = nullptr, *xc = nullptr, *yc = nullptr;
     // imaginary time: two slots of per-wave partial sums (one per step parity),
     // their counts, and one divisor for host-side readers (divisor_kernel)

@@ -250,8 +250,11 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
     }
 }
 
+// FP32 with E as one phase plane and one scaled form on both axes: at most 128
+// VGPRs (4 waves per SIMD; the compiler fits them without spills).  The host
+// sizes the segments for the occupancy of the instance it launches.
 template <typename T, int VEC, bool LHY, int MODE, bool PH = false>
-__global__ void __launch_bounds__(256) step2_kernel(StepParams<T> p) {
+__global__ void __launch_bounds__(256, (sizeof(T) == 4 && PH && MODE >= 0) ? 4 : 1) step2_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;

@@ -209,9 +209,11 @@ __device__ __forceinline__ void fpair_q(T k, T pr, T pi, T &qr, T &qi) {
 
 // Neighbour-lane values through DPP whole-wave shifts by one lane (no LDS
 // traffic): lane i gets lane i+1's value (from_next) or lane i-1's (from_prev);
-// the lane without a neighbour gets 0, which only ever reaches apron columns.
-__device__ __forceinline__ int dpp_from_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_from_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+// the lane without a neighbour gets 0 (bound control: no old value to set
+// first, so the move can also fold into the instruction that reads it), which
+// only ever reaches apron columns.
+__device__ __forceinline__ int dpp_from_next(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_from_prev(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
 __device__ __forceinline__ float shfl_from_next(float v) {
     return __builtin_bit_cast(float, dpp_from_next(__builtin_bit_cast(int, v)));
 }

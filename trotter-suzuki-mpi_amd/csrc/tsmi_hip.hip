@@ -315,8 +315,39 @@ int exchange_halo(tsmi_ctx *c, void *base_re, void *base_im) {
 }
 
 // Two real-time steps in one pass over the whole allocation (fused_kernel.hpp).
+// Segments of the fused pass sized to one round of resident waves of the
+// instance that runs (its registers, hence its occupancy, depend on the scaled
+// form and on the E form); recomputed when either changes.
+template <typename T, int VEC, bool LHY, int MODE, bool PH>
+int size_fused(tsmi_ctx *c) {
+    const int key = (MODE + 1) * 2 + (PH ? 1 : 0);
+    if (c->geo2_key == key) return TSMI_OK;
+    const int sw2 = kWave * VEC - 4 * kApron;
+    c->n_strips2 = (c->Wd + sw2 - 1) / sw2;
+    int seg2 = std::max<int>(16, (c->Hd + 1) / 2);
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step2_kernel<T, VEC, LHY, MODE, PH>, kBlock, 0) ==
+            hipSuccess &&
+        cus > 0 && per_cu > 0) {
+        const long resident = (long)cus * per_cu * (kBlock / kWave);
+        const long segs = std::max<long>(1, resident / c->n_strips2);
+        seg2 = (int)std::max<long>(16, (c->Hd + segs - 1) / segs);
+    }
+    if (c->d.tile_rows > 0) seg2 = c->d.tile_rows;
+    seg2 = std::min(seg2, max_seg_rows(c->pitch));
+    seg2 = (seg2 + 1) / 2 * 2;
+    c->seg_rows2 = std::min<int>(seg2, c->Hd);
+    c->n_segs2 = (c->Hd + c->seg_rows2 - 1) / c->seg_rows2;
+    c->n_waves2 = c->n_strips2 * c->n_segs2;
+    c->geo2_key = key;
+    return TSMI_OK;
+}
+
 template <typename T, int VEC, bool LHY, int MODE, bool PH>
 int launch_step2(tsmi_ctx *c) {
+    if (int st = size_fused<T, VEC, LHY, MODE, PH>(c)) return st;
     StepParams<T> p;
     std::memset(&p, 0, sizeof(p));
     const int nb = 1 - c->cur;
@@ -360,16 +391,6 @@ int step2_once(tsmi_ctx *c) {
     return lhy ? step2_mode<double, 2, true>(c) : step2_mode<double, 2, false>(c);
 }
 
-template <typename T, int VEC, bool LHY>
-hipError_t occupancy2_of(int *blocks) {
-    // sized for the phase form (every E of unit modulus), the one the steps run
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, step2_kernel<T, VEC, LHY, 0, true>, kBlock, 0);
-}
-
-hipError_t occupancy2_blocks(int precision, bool lhy, int *blocks) {
-    if (precision == 32) return lhy ? occupancy2_of<float, kVec32F, true>(blocks) : occupancy2_of<float, kVec32F, false>(blocks);
-    return lhy ? occupancy2_of<double, 2, true>(blocks) : occupancy2_of<double, 2, false>(blocks);
-}
 
 template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH>
 int launch_step(tsmi_ctx *c, int exchange) {
@@ -667,7 +688,6 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     c->sweeps = sweeps;
     c->esz = d.precision == 32 ? 4 : 8;
     c->vec = c->esz == 4 ? (d.imag_time ? kVec32I : kVec32) : 2;
-    const int vec2 = c->esz == 4 ? kVec32F : 2;
     const int sw = kWave * c->vec - 2 * kApron;
     c->n_strips = (c->W + sw - 1) / sw;
     int seg = d.tile_rows;
@@ -695,28 +715,6 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     c->n_waves = c->n_strips * c->n_segs;
     c->ix0 = d.periodic_x ? d.halo_x : 0; c->ix1 = c->W - c->ix0;
     c->iy0 = d.periodic_y ? d.halo_y : 0; c->iy1 = c->H - c->iy0;
-    if (sweeps == 2) {
-        // fused passes: strips of kWave*vec - 16 columns over the whole allocation,
-        // segments sized to one round of resident waves of the fused kernel
-        const int sw2 = kWave * vec2 - 4 * kApron;
-        c->n_strips2 = (c->Wd + sw2 - 1) / sw2;
-        int seg2 = std::max<int>(16, (int)((Hd + 1) / 2));
-        int dev = 0, cus = 0, per_cu = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            occupancy2_blocks(d.precision, d.coupling_lhy != 0.0, &per_cu) == hipSuccess && cus > 0 && per_cu > 0) {
-            const long resident = (long)cus * per_cu * (kBlock / kWave);
-            const long segs = std::max<long>(1, resident / c->n_strips2);
-            seg2 = (int)std::max<long>(16, (Hd + segs - 1) / segs);
-        }
-        if (d.tile_rows > 0) seg2 = d.tile_rows;
-        seg2 = std::min(seg2, max_seg_rows(pitch));
-        seg2 = (seg2 + 1) / 2 * 2;
-        c->seg_rows2 = std::min<int>(seg2, c->Hd);
-        c->n_segs2 = (c->Hd + c->seg_rows2 - 1) / c->seg_rows2;
-        c->n_waves2 = c->n_strips2 * c->n_segs2;
-    }
-
     // kinetic and nonlinear coefficients in FP64 (src/cpukernel.cpp:195-216) and
     // the kernel's scaled-frame factors
     const StepCoefs k = step_coefs(d, c->H, c->esz);
