@@ -53,6 +53,10 @@ CASES = [
     dict(n_x=8, n_y=8, periodic_x=True, periodic_y=True, steps=9, g=0.5),            # smallest periodic
     dict(n_x=37, n_y=1, periodic_x=False, periodic_y=False, steps=9),                # single row
     dict(n_x=40, n_y=33, periodic_x=True, periodic_y=False, imag_time=True, steps=25, g=4.0),
+    # interior strips with segments long enough for the steady-state loop, where each
+    # row is renormalised as it enters the pipeline
+    dict(n_x=520, n_y=200, periodic_x=False, periodic_y=False, imag_time=True, steps=20, g=10.0),
+    dict(n_x=500, n_y=96, periodic_x=True, periodic_y=True, imag_time=True, steps=15, g=5.0, g_lhy=1.0),
 ]
 
 
@@ -113,7 +117,8 @@ def test_fp64_matches_reference_golden(name, oracle):
     np.testing.assert_allclose(x, z["expect"], rtol=1e-11, atol=1e-13)
 
 
-@pytest.mark.parametrize("case", [CASES[1], CASES[4], CASES[5], CASES[8]], ids=lambda c: case_id(c))
+@pytest.mark.parametrize("case", [CASES[1], CASES[4], CASES[5], CASES[8], CASES[14], CASES[15]],
+                         ids=lambda c: case_id(c))
 def test_fp32_bulk_within_tolerance(case, oracle):
     """FP32 bulk with FP64 coefficients rounded once and FP64 reductions:
     relative L2 below 2e-5 after up to 100 steps on these cases."""

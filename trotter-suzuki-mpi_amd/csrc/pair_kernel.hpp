@@ -268,8 +268,8 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
                     const Row<T, VEC> &eb, T (&na)[VEC], T (&nb)[VEC]) {
         constexpr int TP = decltype(tp)::value;
         const bool v0 = valid(t), v1 = valid(t - 1), v2 = valid(t - 2), v3 = valid(t - 3), v4 = valid(t - 4);
-        if (div0_on && v0) scale_row<T, VEC>(ia, div0, inv0);
-        if (div1_on && v0) scale_row<T, VEC>(ib, div1, inv1);
+        if (div0_on && v0) scale_row<T, VEC>(ia, inv0);
+        if (div1_on && v0) scale_row<T, VEC>(ib, inv1);
         T ta[VEC], tb[VEC];
         density(ia, ta);
         density(ib, tb);

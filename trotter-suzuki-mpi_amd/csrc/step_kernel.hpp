@@ -102,22 +102,17 @@ __device__ __forceinline__ double renorm_divisor(const double *part, int n, doub
     return sqrt(nrm / target);
 }
 
-// x / d for a divisor d with inv = 1 / d (both FP64): the product and one
-// fused correction (Markstein), which gives the correctly rounded quotient at
-// the cost of three multiply-adds instead of a full division.
-__device__ __forceinline__ double div_by(double x, double d, double inv) {
-    const double q = x * inv;
-    return fma(fma(-q, d, x), inv, q);
-}
-
-// Divides a row by the pending renormalisation divisor: the FP64 quotient,
-// rounded once to the row's precision.
+// Divides a row by the pending renormalisation divisor d: a multiplication by
+// 1/d in the row's precision.  The reciprocal's rounding scales the whole state
+// by one factor (within half an ulp of 1), which the next renormalisation
+// removes; each element is rounded once, as a quotient would be.
 template <typename T, int VEC>
-__device__ __forceinline__ void scale_row(Row<T, VEC> &r, double d, double inv) {
+__device__ __forceinline__ void scale_row(Row<T, VEC> &r, double inv) {
+    const T s = T(inv);
 #pragma unroll
     for (int k = 0; k < VEC; ++k) {
-        r.re[k] = T(div_by(double(r.re[k]), d, inv));
-        r.im[k] = T(div_by(double(r.im[k]), d, inv));
+        r.re[k] *= s;
+        r.im[k] *= s;
     }
 }
 
@@ -661,19 +656,19 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     auto valid = [&](int y) { return y >= y_lo && y < y_hi; };
 
     // Writes the row leaving the pipeline.
-    // Imaginary time: adds the inner cells of an output row to the sum.
+    // Imaginary time: adds the inner cells of an output row to the sum (the
+    // lane's few cells of the row summed first, then one compensated step).
     auto accumulate = [&](int y, const Row<T, VEC> &o4) {
         if (IMAG && y >= p.iy0 && y < p.iy1) {
+            double v = 0.0;
 #pragma unroll
             for (int k = 0; k < VEC; ++k) {
-                if (inner_mask & (1u << k)) {
-                    const double vr = double(o4.re[k]), vi = double(o4.im[k]);
-                    const double v = vr * vr + vi * vi;
-                    const double t2 = acc + v;
-                    comp += (fabs(acc) >= fabs(v)) ? (acc - t2) + v : (v - t2) + acc;
-                    acc = t2;
-                }
+                const double vr = double(o4.re[k]), vi = double(o4.im[k]);
+                if (inner_mask & (1u << k)) v += vr * vr + vi * vi;
             }
+            const double t2 = acc + v;
+            comp += (fabs(acc) >= fabs(v)) ? (acc - t2) + v : (v - t2) + acc;
+            acc = t2;
         }
     };
     auto emit = [&](int y, const Row<T, VEC> &o4) {
@@ -702,7 +697,7 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         const T a2 = (v1 && v2) ? mV : unV, b2 = (v1 && v2) ? tV : T(0);
         const T a3 = (v2 && v3) ? mV : unV, b3 = (v2 && v3) ? tV : T(0);
         const T a4 = (v3 && v4) ? mV : unV, b4 = (v3 && v4) ? tV : T(0);
-        if (IMAG && scale_in && v0) scale_row<T, VEC>(in, divd, dinv);
+        if (IMAG && scale_in && v0) scale_row<T, VEC>(in, dinv);
         // stage 1: V0, then H0 on row t-1
         vpair<(TP + 1) & 1, IMAG>(pend1, in, a1, b1);
         Row<T, VEC> o1 = pend1;
@@ -733,7 +728,7 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     auto beat_fast = [&](auto tp, int t, Row<T, VEC> &in, const Row<T, VEC> &e) {
         constexpr int TP = decltype(tp)::value;
         constexpr int M = MODE < 0 ? 0 : MODE;
-        if (IMAG && scale_in) scale_row<T, VEC>(in, divd, dinv);
+        if (IMAG && scale_in) scale_row<T, VEC>(in, dinv);
         vpair_fast<(TP + 1) & 1, IMAG, M>(pend1, in, kV);
         Row<T, VEC> o1 = pend1;
         pend1 = in;
