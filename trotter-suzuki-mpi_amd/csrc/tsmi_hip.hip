@@ -695,8 +695,12 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
         // Size the launch to one round of resident wavefronts: as many row
         // segments per strip as the device holds at this kernel's occupancy, so
         // no partial second round idles most of the GPU; on small lattices at
-        // least 24 rows, below which the 4-row aprons cost more than the extra
-        // waves bring (measured on 2048^2 and 1024^2).
+        // least 24 rows, below which the 4-row aprons and the per-wave partial
+        // sums cost more than the extra waves bring -- unless 24-row segments
+        // leave SIMDs without a wave, where a wave's row latency bounds the step
+        // and 12-row segments are faster (measured, imaginary time: 1024^2 FP32
+        // 28.9 -> 22.5 us, FP64 30.3 -> 23.2; 2048^2 FP32 33.7 -> 31.3; FP64 at
+        // 2048^2, one wave per SIMD at 24 rows, 50.0 vs 52.3 with 12).
         seg = std::max<int>(24, (int)((H + 1) / 2));
         int dev = 0, cus = 0, blocks_per_cu = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -706,6 +710,9 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
             const long resident = (long)cus * blocks_per_cu * (kBlock / kWave);
             const long segs = std::max<long>(1, resident / c->n_strips);
             seg = (int)std::max<long>(24, (H + segs - 1) / segs);
+            const long simds = 4L * cus;
+            if ((long)c->n_strips * ((H + seg - 1) / seg) < simds)
+                seg = (int)std::max<long>(12, (H + segs - 1) / segs);
         }
     }
     seg = std::min(seg, max_seg_rows(pitch));
