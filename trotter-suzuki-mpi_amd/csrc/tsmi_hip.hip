@@ -314,7 +314,6 @@ int exchange_halo(tsmi_ctx *c, void *base_re, void *base_im) {
     return TSMI_OK;
 }
 
-// Two real-time steps in one pass over the whole allocation (fused_kernel.hpp).
 // Segments of the fused pass sized to one round of resident waves of the
 // instance that runs (its registers, hence its occupancy, depend on the scaled
 // form and on the E form); recomputed when either changes.
@@ -345,6 +344,7 @@ int size_fused(tsmi_ctx *c) {
     return TSMI_OK;
 }
 
+// Two real-time steps in one pass over the whole allocation (fused_kernel.hpp).
 template <typename T, int VEC, bool LHY, int MODE, bool PH>
 int launch_step2(tsmi_ctx *c) {
     if (int st = size_fused<T, VEC, LHY, MODE, PH>(c)) return st;
@@ -390,7 +390,6 @@ int step2_once(tsmi_ctx *c) {
     if (c->esz == 4) return lhy ? step2_mode<float, kVec32F, true>(c) : step2_mode<float, kVec32F, false>(c);
     return lhy ? step2_mode<double, 2, true>(c) : step2_mode<double, 2, false>(c);
 }
-
 
 template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH>
 int launch_step(tsmi_ctx *c, int exchange) {
