@@ -291,13 +291,61 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
     if (t < y_hi) load_psi(t, ia0, ib0);
     if (t + 1 < y_hi) load_psi(t + 1, ia1, ib1);
     const int t_end = y_hi + 4;
-    for (; t < t_end; t += 2) {
+    auto pair_general = [&](int t) {
         beat(I0(), t, ia0, ib0, ea0, eb0, na0, nb0);
         if (t + 2 < y_hi) load_psi(t + 2, ia0, ib0);
         if (t < y_hi) load_e(t, ea0, eb0);
         beat(I1(), t + 1, ia1, ib1, ea1, eb1, na1, nb1);
         if (t + 3 < y_hi) load_psi(t + 3, ia1, ib1);
         if (t + 1 < y_hi) load_e(t + 1, ea1, eb1);
+    };
+    if constexpr (sizeof(T) == 8) {
+        // FP64: the steady state (t >= y_lo + 8, t + 3 < y_hi: rows t-4..t+3 in
+        // the segment, the emitted rows in the output range) in a loop of its own
+        // with every validity known and no conditional memory operation, which
+        // lets the compiler fit the pass in 254 VGPRs (2 waves per SIMD; the
+        // single loop needed 256 plus AGPR spills, 1 wave): 0.350 -> 0.324 ms per
+        // step at 4096^2.
+        auto beat_steady = [&](auto tp, int t, Row<T, VEC> &ia, Row<T, VEC> &ib, const Row<T, VEC> &ea,
+                               const Row<T, VEC> &eb, T (&na)[VEC], T (&nb)[VEC]) {
+            constexpr int TP = decltype(tp)::value;
+            if (div0_on) scale_row<T, VEC>(ia, inv0);
+            if (div1_on) scale_row<T, VEC>(ib, inv1);
+            T ta[VEC], tb[VEC];
+            density(ia, ta);
+            density(ib, tb);
+            Row<T, VEC> oa, ob;
+            pair_pipe<TP, IMAG, PH>(qa, ia, ea, nb, oa, true, true, true, true, true, hc0, mV0, tV0, unV0, c0, cab,
+                                    mag0);
+            pair_pipe<TP, IMAG, PH>(qb, ib, eb, na, ob, true, true, true, true, true, hc1, mV1, tV1, unV1, c1, cab,
+                                    mag1);
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                na[k] = ta[k];
+                nb[k] = tb[k];
+            }
+            emit(t - 4, oa, ob);
+        };
+        for (; t < t_end && t < y_lo + 8; t += 2) pair_general(t);
+        for (; t + 3 < y_hi; t += 2) {
+            beat_steady(I0(), t, ia0, ib0, ea0, eb0, na0, nb0);
+            load_psi(t + 2, ia0, ib0);
+            load_e(t, ea0, eb0);
+            beat_steady(I1(), t + 1, ia1, ib1, ea1, eb1, na1, nb1);
+            load_psi(t + 3, ia1, ib1);
+            load_e(t + 1, ea1, eb1);
+        }
+        for (; t < t_end; t += 2) pair_general(t);
+    } else {
+        // FP32: one loop (measured faster than the split one in FP32)
+        for (; t < t_end; t += 2) {
+            beat(I0(), t, ia0, ib0, ea0, eb0, na0, nb0);
+            if (t + 2 < y_hi) load_psi(t + 2, ia0, ib0);
+            if (t < y_hi) load_e(t, ea0, eb0);
+            beat(I1(), t + 1, ia1, ib1, ea1, eb1, na1, nb1);
+            if (t + 3 < y_hi) load_psi(t + 3, ia1, ib1);
+            if (t + 1 < y_hi) load_e(t + 1, ea1, eb1);
+        }
     }
 
     if (IMAG) {
