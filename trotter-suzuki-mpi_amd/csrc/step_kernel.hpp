@@ -60,14 +60,15 @@ struct StepParams {
     T mH, tH, mV, tV, unH, unV;
     T c_nl, c_lhy;               // nonlinear couplings, already rescaled to the frame
     // Imaginary time only.  The renormalisation of a step's output is applied
-    // as the next step loads it: every wave of a step stores its FP64 partial
-    // sum of |psi|^2, wave 0 stores their count, and every wave of the next
-    // launch sums them in wave order (renorm_divisor).  The launch boundary makes
-    // the stores visible; no wave waits for another inside a launch.
+    // as the next step loads it: every wave of a step forms its FP64 partial
+    // sum of |psi|^2, stored per wave (batches) or combined per workgroup
+    // (step_kernel), with their count, and every wave of the next launch sums
+    // them in order (renorm_divisor).  The launch boundary makes the stores
+    // visible; no wave waits for another workgroup inside a launch.
     const double *part_in;       // partial sums of the step that wrote the input (nullptr: divisor 1)
     const int *n_in;             // their count (0: divisor 1)
-    double *part_out;            // this step's partial sums, one per wave
-    int *n_out;                  // wave 0 stores n_waves here
+    double *part_out;            // this step's partial sums, one per wave or per workgroup
+    int *n_out;                  // and their count
     double norm_target, dxdy;
     int ix0, ix1, iy0, iy1;      // inner region (tile coordinates)
 };
@@ -578,8 +579,11 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
 // ---------------------------------------------------------------------------
 // One wavefront's share of a step: strip (wave mod n_strips), segment
 // (wave / n_strips).  `wave` is wave-uniform.
-template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false>
-__device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave, const int lane) {
+// BLOCK_SUM (imaginary time): the wave returns its partial sum in *blk instead
+// of storing it, and the caller combines the partials of its workgroup.
+template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false, bool BLOCK_SUM = false>
+__device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave, const int lane,
+                                          double *blk = nullptr) {
     constexpr int SW = kWave * VEC - 2 * kApron;       // output columns per strip
     const int strip = wave % p.n_strips;
     const int seg = wave / p.n_strips;
@@ -830,21 +834,49 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         // this wave's partial sum for the next step's divisor
         double s = acc + comp;
         for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
-        if (lane == 0) {
+        if constexpr (BLOCK_SUM) {
+            *blk = s;
+        } else if (lane == 0) {
             p.part_out[wave] = s;
             if (wave == 0) *p.n_out = p.n_waves;
         }
     }
 }
 
+// One launch of one step.  Imaginary time: the partial sums of a workgroup's
+// waves are combined in wave order into one partial per workgroup (LDS and one
+// barrier at the end), so the next launch's waves each read a quarter as many
+// partials (the read is O(waves x partials) per step).  Every wave of the grid
+// reaches that barrier, those past the last segment included.
 template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false>
 __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform by construction; readfirstlane lets the compiler keep every
     // quantity derived from it (segment, strip, row bounds) in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
-    if (wave >= p.n_waves) return;
-    step_wave<T, IMAG, VEC, LHY, MODE, PH>(p, wave, lane);
+    if constexpr (!IMAG) {
+        if (wave >= p.n_waves) return;
+        step_wave<T, IMAG, VEC, LHY, MODE, PH>(p, wave, lane);
+    } else {
+        constexpr int kWavesPerBlock = 256 / kWave;
+        __shared__ double part[kWavesPerBlock];
+        double s = 0.0;
+        if (wave < p.n_waves) step_wave<T, IMAG, VEC, LHY, MODE, PH, true>(p, wave, lane, &s);
+        if (lane == 0) part[threadIdx.x / kWave] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = 0.0, c = 0.0;
+#pragma unroll
+            for (int i = 0; i < kWavesPerBlock; ++i) {
+                const double v = part[i];
+                const double t = a + v;
+                c += (fabs(a) >= fabs(v)) ? (a - t) + v : (v - t) + a;
+                a = t;
+            }
+            p.part_out[blockIdx.x] = a + c;
+            if (blockIdx.x == 0) *p.n_out = (int)gridDim.x;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
