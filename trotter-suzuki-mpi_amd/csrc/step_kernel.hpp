@@ -791,14 +791,15 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         if (t + 7 < y_hi) load_psi(t + 7, inD);
         if (t + 5 < y_hi) load_e(t + 5, eB);
     };
-    // The steady state of an interior strip (t >= y_lo + 4, t + 3 < y_hi).  No
-    // memory operation is conditional, so the compiler keeps the loads of the
-    // rows ahead in flight across iterations (a merge of paths with different
-    // counts of outstanding operations makes it drain them): a row past the
-    // segment is addressed through the vector offset, which the buffer's range
-    // check turns into a load of zeros (such a row only meets pairs with
-    // coefficients (1, 0)), and a row before the output range (the first four
-    // beats) gets a store offset past the buffer, which drops it.
+    // The steady state of an interior strip (t >= y_lo + 4, t + 3 < y_hi), in a
+    // loop of its own with no conditional memory operation: the single loop
+    // merged fast and general paths with different counts of outstanding
+    // operations, where the compiler drained them, and needed more registers
+    // (FP32: 204 VGPRs against 161).  A row past the segment is addressed
+    // through the vector offset, which the buffer's range check turns into a
+    // load of zeros (such a row only meets pairs with coefficients (1, 0)), and
+    // a row before the output range (the first four beats) gets a store offset
+    // past the buffer, which drops it.
     auto load_psi_any = [&](int y, Row<T, VEC> &r) {
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
         buf_load<T, VEC>(r_in_re, v_ld + so, 0u, r.re);
