@@ -239,8 +239,10 @@ def adjoint_study():
 def pair_study(n: int = 4096, steps: int = 200):
     """Two-component real-time GPE with Rabi coupling on an n x n periodic
     lattice (two seeded Thomas-Fermi clouds, g_a = g_b = 50, g_ab = 40,
-    omega = 1), FP32 and FP64: one pair_kernel pass per step (both components,
-    the Rabi rotation applied at the pass exit) plus the halo exchange."""
+    omega = 1).  FP32: two steps of both components per pass (pair2_kernel, the
+    Rabi rotation applied between and after them; the first and last step of
+    the call run alone), also timed at one step per pass; FP64: one pair_kernel
+    pass per step.  Each pass is followed by the halo exchange."""
     import numpy as np
     from tsmi import workloads as W
     from tsmi.lattice import Lattice2D
@@ -257,8 +259,8 @@ def pair_study(n: int = 4096, steps: int = 200):
     sites = n * n
     out = {"workload": f"two-component real-time GPE with Rabi coupling, {n}x{n} periodic, g_a = g_b = 50, "
                        f"g_ab = 40, omega = 1, dt = 5e-4, {steps} steps", "inner_sites": sites}
-    for prec in (32, 64):
-        with HipPair(pw, precision=prec) as h:
+    for key, prec, sweeps in (("fp32", 32, 0), ("fp32_one_step_per_pass", 32, 1), ("fp64", 64, 0)):
+        with HipPair(pw, precision=prec, sweeps=sweeps) as h:
             h.evolve(4)
             h.synchronize()
             t = time.perf_counter()
@@ -268,7 +270,7 @@ def pair_study(n: int = 4096, steps: int = 200):
             e = h.energy()
             geo = h.geometry()
         alg = 2 * BYTES_PER_SU[prec] * sites * steps
-        out[f"fp{prec}"] = {"ms_per_step": sec * 1e3 / steps,
+        out[key] = {"ms_per_step": sec * 1e3 / steps,
                             "component_site_updates_per_s": 2 * sites * steps / sec,
                             "algorithmic_GB_per_s": alg / sec / 1e9, "norm": e[11], "geometry": geo}
     return out

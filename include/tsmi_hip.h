@@ -183,7 +183,11 @@ int tsmi_batch_synchronize(tsmi_batch *batch);
  * coupling_b; coupling_lhy 0 (the two-state kernel has no LHY term,
  * src/cpukernel.cpp:332-335); a.norm_target / b.norm_target = the squared norms
  * of the two states when the kernel is built (src/solver.cpp:153-159), the
- * imaginary-time targets.  sweeps is ignored (one step per pass). */
+ * imaginary-time targets.  a.sweeps picks the steps per pass of tsmi_pair_evolve:
+ * 0 chooses two wherever they are exact (FP32, real time, both axes closed or both
+ * periodic, and both E tables of unit modulus; otherwise one), 1 forces one, 2
+ * requires a lattice and precision where two can run.
+ * b.sweeps is not read. */
 typedef struct {
     tsmi_desc a, b;
     double coupling_ab;              /* Hamiltonian2Component::coupling_ab      */

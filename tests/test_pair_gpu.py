@@ -38,10 +38,10 @@ def make(case):
     return W.pair_case(**case, **COUPLED)
 
 
-def run_gpu(oracle, pw, precision, initial_half=True, tile_rows=0):
+def run_gpu(oracle, pw, precision, initial_half=True, tile_rows=0, sweeps=0):
     from tsmi.pair import HipPair
     e = oracle.pair_exp_potentials(pw)
-    with HipPair(pw, precision=precision, e_planes=e, tile_rows=tile_rows) as h:
+    with HipPair(pw, precision=precision, e_planes=e, tile_rows=tile_rows, sweeps=sweeps) as h:
         for n in pw.calls:
             h.evolve(n, initial_half=initial_half)
         return h.state(0), h.state(1), h.energy()
@@ -82,6 +82,34 @@ def test_pair_segments_do_not_change_the_result(oracle):
     a1, b1, _ = run_gpu(oracle, pw, 64, tile_rows=8)
     a2, b2, _ = run_gpu(oracle, pw, 64, tile_rows=0)
     assert rel_l2(a1, a2) < 1e-14 and rel_l2(b1, b2) < 1e-14
+
+
+FUSED = [CASES[0], CASES[1], CASES[4], CASES[5], CASES[6]]
+
+
+@pytest.mark.parametrize("tile_rows", [0, 16])
+@pytest.mark.parametrize("case", FUSED, ids=[case_id(c) for c in FUSED])
+def test_pair_two_steps_per_pass_fp32(oracle, case, tile_rows):
+    """FP32 real time on closed or periodic lattices, two steps per pass
+    (pair2_kernel; on periodic axes the first and last step of each call run
+    alone): the same state as one step per pass, and the oracle's."""
+    pw = make(case)
+    print(f"[pair] fp32 two steps per pass {case_id(case)} tile_rows={tile_rows}", flush=True)
+    a1, b1, _ = run_gpu(oracle, pw, 32, tile_rows=tile_rows, sweeps=1)
+    a2, b2, _ = run_gpu(oracle, pw, 32, tile_rows=tile_rows, sweeps=2)
+    ra, rb = oracle.evolve2(pw)
+    assert rel_l2(a2, a1) < 1e-6 and rel_l2(b2, b1) < 1e-6
+    assert rel_l2(a2, ra) < 2e-5 and rel_l2(b2, rb) < 2e-5
+
+
+@pytest.mark.parametrize("precision,case", [(64, CASES[1]), (32, CASES[2]), (32, CASES[8])],
+                         ids=["fp64", "fp32_mixed_axes", "fp32_imag"])
+def test_pair_two_steps_per_pass_refused(precision, case):
+    """sweeps 2 where two steps per pass are not exact or not built is an error."""
+    from tsmi import _lib
+    from tsmi.pair import HipPair
+    with pytest.raises(_lib.TsmiError, match="two steps per pass"):
+        HipPair(make(case), precision=precision, sweeps=2)
 
 
 def test_mixed_bec_imaginary_known_answer(oracle):

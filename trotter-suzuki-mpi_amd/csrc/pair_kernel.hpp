@@ -1,4 +1,5 @@
-// pair_kernel.hpp -- both components of a two-component lattice, one step per pass.
+// pair_kernel.hpp -- both components of a two-component lattice, one step per
+// pass (pair_kernel) or, in FP32 real time, two (pair2_kernel).
 //
 // The reference advances a two-component system (Hamiltonian2Component with the
 // two-state CPUBlock, src/cpukernel.cpp:274-392) by running the single-component
@@ -40,6 +41,7 @@ struct PairParams {
     // precision: R = [[c, i s], [i conj(s), c]] has R^H R = (c^2 + |s|^2) I, so
     // constants rounded to FP32 would scale the norm by the same factor every step.
     double cc, cs_r, cs_i;
+    double cc2, cs_r2, cs_i2;        // the second step's rotation in a two-step pass (pair2_kernel)
     int rabi;                        // 1: apply it
     // imaginary time
     const double *div_in[2];         // pending divisors of the inputs (nullptr: 1)
@@ -131,6 +133,20 @@ __device__ __forceinline__ void pair_pipe(Pipe4<T, VEC> &q, Row<T, VEC> &in, con
     vpair<TP & 1, IMAG>(q.p4, o3, a4, b4);
     o4 = q.p4;
     q.p4 = o3;
+}
+
+// The real-time Rabi rotation of a pair of rows (src/cpucartesian.cpp:264-276,
+// every right-hand side the old value), in FP64.
+template <typename T, int VEC>
+__device__ __forceinline__ void rabi_rows(Row<T, VEC> &oa, Row<T, VEC> &ob, double cc, double sr, double si) {
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        const double ar = oa.re[k], ai = oa.im[k], br = ob.re[k], bi = ob.im[k];
+        oa.re[k] = T(cc * ar - si * br - sr * bi);
+        oa.im[k] = T(cc * ai + sr * br - si * bi);
+        ob.re[k] = T(cc * br + si * ar - sr * ai);
+        ob.im[k] = T(cc * bi + sr * ar + si * ai);
+    }
 }
 
 template <typename T, int VEC>
@@ -231,18 +247,7 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
     auto valid = [&](int y) { return y >= y_lo && y < y_hi; };
 
     auto emit = [&](int y, Row<T, VEC> &oa, Row<T, VEC> &ob) {
-        if (!IMAG && p.rabi) {
-            // src/cpucartesian.cpp:264-276, every right-hand side the old value
-            const double cc = p.cc, sr = p.cs_r, si = p.cs_i;
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) {
-                const double ar = oa.re[k], ai = oa.im[k], br = ob.re[k], bi = ob.im[k];
-                oa.re[k] = T(cc * ar - si * br - sr * bi);
-                oa.im[k] = T(cc * ai + sr * br - si * bi);
-                ob.re[k] = T(cc * br + si * ar - sr * ai);
-                ob.im[k] = T(cc * bi + sr * ar + si * ai);
-            }
-        }
+        if (!IMAG && p.rabi) rabi_rows<T, VEC>(oa, ob, p.cc, p.cs_r, p.cs_i);
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
         buf_store<T, VEC>(ro0r, v_st, so, oa.re);
         buf_store<T, VEC>(ro0i, v_st, so, oa.im);
@@ -394,6 +399,206 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) pair_kernel(PairP
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
     pair_wave<T, IMAG, VEC, PH>(p, wave, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Two real-time steps of both components per pass
+// ---------------------------------------------------------------------------
+// pair_wave's two component pipelines, twice in series with an 8-cell apron (the
+// dependency cone of two steps), as step2_wave chains two single-component
+// pipelines: the first pair (A) runs step n on the input rows; every pair of rows
+// it emits (4 rows behind its input) is Rabi-rotated by the first step's
+// constants and enters the second pair (B), which runs step n+1 and emits 8 rows
+// behind the input, rotated by the second step's constants.  B's potential
+// passes see the densities of B's input rows -- the start-of-step densities of
+// step n+1, after the rotation, as in the reference, where the rotation ends a
+// step (src/solver.cpp:201-226).  Per site a pass reads psi_a, psi_b once, the E
+// rows twice (mostly from the caches) and writes psi_a, psi_b once.  Exactness
+// across tiles follows the single-component pass: closed axes, or periodic axes
+// with the 8-wide halo refreshed before each pass and E wrapped over it.
+template <typename T, int VEC, bool PH>
+__device__ __forceinline__ void pair2_wave(const PairParams<T> &p, const int wave, const int lane) {
+    constexpr int AP = 2 * kApron;
+    constexpr int SW = kWave * VEC - 2 * AP;
+    const int strip = wave % p.n_strips;
+    const int seg = wave / p.n_strips;
+    const int X0 = strip * SW - AP;
+    const int xl = X0 + lane * VEC;
+    const int y_out_lo = seg * p.seg_rows;
+    const int y_out_hi = min(p.H, y_out_lo + p.seg_rows);
+    const int y_lo = max(0, y_out_lo - AP);
+    const int y_hi = min(p.H, y_out_hi + AP);
+    const bool ld_ok = xl >= 0 && xl + VEC <= p.pitch;
+    const bool own = lane >= AP / VEC && lane < kWave - AP / VEC;
+
+    const HCoef<T, VEC> hc0 = make_hcoef<T, VEC>(xl, lane, p.W, p.mH[0], p.tH[0], p.unH[0]);
+    const HCoef<T, VEC> hc1 = make_hcoef<T, VEC>(xl, lane, p.W, p.mH[1], p.tH[1], p.unH[1]);
+    const T mV0 = p.mV[0], tV0 = p.tV[0], unV0 = p.unV[0], c0 = p.c_self[0];
+    const T mV1 = p.mV[1], tV1 = p.tV[1], unV1 = p.unV[1], c1 = p.c_self[1];
+    const T cab = p.c_ab;
+    const double mag0 = p.mag[0], mag1 = p.mag[1];
+    const bool rabi = p.rabi != 0;
+
+    const long seg0 = (long)y_lo * p.pitch;
+    const unsigned seg_bytes = (unsigned)((long)(y_hi - y_lo) * p.pitch * (long)sizeof(T));
+    const __amdgpu_buffer_rsrc_t ri0r = make_rsrc(p.in_re[0] + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t ri0i = make_rsrc(p.in_im[0] + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t ri1r = make_rsrc(p.in_re[1] + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t ri1i = make_rsrc(p.in_im[1] + seg0, seg_bytes);
+    const T *e0 = PH ? p.ph[0] : p.e_re[0], *e1 = PH ? p.ph[1] : p.e_re[1];
+    const __amdgpu_buffer_rsrc_t re0r = make_rsrc(e0 + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t re0i = make_rsrc(PH ? e0 + seg0 : p.e_im[0] + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t re1r = make_rsrc(e1 + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t re1i = make_rsrc(PH ? e1 + seg0 : p.e_im[1] + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t ro0r = make_rsrc(p.out_re[0] + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t ro0i = make_rsrc(p.out_im[0] + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t ro1r = make_rsrc(p.out_re[1] + seg0, seg_bytes);
+    const __amdgpu_buffer_rsrc_t ro1i = make_rsrc(p.out_im[1] + seg0, seg_bytes);
+    const unsigned row_bytes = (unsigned)p.pitch * (unsigned)sizeof(T);
+    const unsigned v_ld = ld_ok ? (unsigned)xl * (unsigned)sizeof(T) : kOffNone;
+    const unsigned v_st = (own && ld_ok) ? (unsigned)xl * (unsigned)sizeof(T) : kOffNone;
+
+    auto load_psi = [&](int y, Row<T, VEC> &a, Row<T, VEC> &b) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_load<T, VEC>(ri0r, v_ld, so, a.re);
+        buf_load<T, VEC>(ri0i, v_ld, so, a.im);
+        buf_load<T, VEC>(ri1r, v_ld, so, b.re);
+        buf_load<T, VEC>(ri1i, v_ld, so, b.im);
+    };
+    // a state row past the segment through the vector offset: the buffer's
+    // range check turns it into a load of zeros
+    auto load_psi_any = [&](int y, Row<T, VEC> &a, Row<T, VEC> &b) {
+        const unsigned off = v_ld + (unsigned)(y - y_lo) * row_bytes;
+        buf_load<T, VEC>(ri0r, off, 0u, a.re);
+        buf_load<T, VEC>(ri0i, off, 0u, a.im);
+        buf_load<T, VEC>(ri1r, off, 0u, b.re);
+        buf_load<T, VEC>(ri1i, off, 0u, b.im);
+    };
+    auto load_e = [&](int y, Row<T, VEC> &a, Row<T, VEC> &b) {
+        const unsigned so = (unsigned)(y - y_lo) * row_bytes;
+        buf_load<T, VEC>(re0r, v_ld, so, a.re);
+        buf_load<T, VEC>(re1r, v_ld, so, b.re);
+        if (!PH) {
+            buf_load<T, VEC>(re0i, v_ld, so, a.im);
+            buf_load<T, VEC>(re1i, v_ld, so, b.im);
+        }
+    };
+    // the second step's rotation, then the stores; a row before the output range
+    // gets an offset past the buffer, which drops the store
+    auto emit_any = [&](int y, Row<T, VEC> &xa, Row<T, VEC> &xb) {
+        if (rabi) rabi_rows<T, VEC>(xa, xb, p.cc2, p.cs_r2, p.cs_i2);
+        const unsigned off = y >= y_out_lo ? v_st + (unsigned)(y - y_lo) * row_bytes : kOffNone;
+        buf_store<T, VEC>(ro0r, off, 0u, xa.re);
+        buf_store<T, VEC>(ro0i, off, 0u, xa.im);
+        buf_store<T, VEC>(ro1r, off, 0u, xb.re);
+        buf_store<T, VEC>(ro1i, off, 0u, xb.im);
+    };
+
+    Pipe4<T, VEC> A0, A1, B0, B1;
+    // input rows, E rows of pair A (row t-2) and of pair B (row t-6), and the
+    // start-of-step densities of each pair's input rows, in slots by row parity
+    Row<T, VEC> ia0, ib0, ia1, ib1, ea0, eb0, ea1, eb1, fa0, fb0, fa1, fb1;
+    T na0[VEC], nb0[VEC], na1[VEC], nb1[VEC], ma0[VEC], mb0[VEC], ma1[VEC], mb1[VEC];
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        A0.p1.re[k] = A0.p1.im[k] = A0.p2.re[k] = A0.p2.im[k] = A0.p3.re[k] = A0.p3.im[k] = T(0);
+        A0.p4.re[k] = A0.p4.im[k] = A1.p1.re[k] = A1.p1.im[k] = A1.p2.re[k] = A1.p2.im[k] = T(0);
+        A1.p3.re[k] = A1.p3.im[k] = A1.p4.re[k] = A1.p4.im[k] = T(0);
+        B0.p1.re[k] = B0.p1.im[k] = B0.p2.re[k] = B0.p2.im[k] = B0.p3.re[k] = B0.p3.im[k] = T(0);
+        B0.p4.re[k] = B0.p4.im[k] = B1.p1.re[k] = B1.p1.im[k] = B1.p2.re[k] = B1.p2.im[k] = T(0);
+        B1.p3.re[k] = B1.p3.im[k] = B1.p4.re[k] = B1.p4.im[k] = T(0);
+        ia0.re[k] = ia0.im[k] = ib0.re[k] = ib0.im[k] = ia1.re[k] = ia1.im[k] = ib1.re[k] = ib1.im[k] = T(0);
+        ea0.re[k] = ea0.im[k] = eb0.re[k] = eb0.im[k] = ea1.re[k] = ea1.im[k] = eb1.re[k] = eb1.im[k] = T(0);
+        fa0.re[k] = fa0.im[k] = fb0.re[k] = fb0.im[k] = fa1.re[k] = fa1.im[k] = fb1.re[k] = fb1.im[k] = T(0);
+        na0[k] = nb0[k] = na1[k] = nb1[k] = ma0[k] = mb0[k] = ma1[k] = mb1[k] = T(0);
+    }
+    auto valid = [&](int y) { return y >= y_lo && y < y_hi; };
+
+    // beat t with row validities v[k] = valid(t - k): pair A takes input row t
+    // and gives row t-4, pair B takes it and gives row t-8 (emitted when wanted)
+    auto beat = [&](auto tp, int t, bool v0, bool v1, bool v2, bool v3, bool v4, bool v5, bool v6, bool v7, bool v8,
+                    bool want, Row<T, VEC> &ia, Row<T, VEC> &ib, const Row<T, VEC> &ea, const Row<T, VEC> &eb,
+                    const Row<T, VEC> &fa, const Row<T, VEC> &fb, T (&na)[VEC], T (&nb)[VEC], T (&ma)[VEC],
+                    T (&mb)[VEC]) {
+        constexpr int TP = decltype(tp)::value;
+        T ta[VEC], tb[VEC];
+        density(ia, ta);
+        density(ib, tb);
+        Row<T, VEC> oa, ob;
+        pair_pipe<TP, false, PH>(A0, ia, ea, nb, oa, v0, v1, v2, v3, v4, hc0, mV0, tV0, unV0, c0, cab, mag0);
+        pair_pipe<TP, false, PH>(A1, ib, eb, na, ob, v0, v1, v2, v3, v4, hc1, mV1, tV1, unV1, c1, cab, mag1);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            na[k] = ta[k];
+            nb[k] = tb[k];
+        }
+        if (rabi) rabi_rows<T, VEC>(oa, ob, p.cc, p.cs_r, p.cs_i);
+        density(oa, ta);
+        density(ob, tb);
+        Row<T, VEC> xa, xb;
+        pair_pipe<TP, false, PH>(B0, oa, fa, mb, xa, v4, v5, v6, v7, v8, hc0, mV0, tV0, unV0, c0, cab, mag0);
+        pair_pipe<TP, false, PH>(B1, ob, fb, ma, xb, v4, v5, v6, v7, v8, hc1, mV1, tV1, unV1, c1, cab, mag1);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            ma[k] = ta[k];
+            mb[k] = tb[k];
+        }
+        if (want) emit_any(t - 8, xa, xb);
+    };
+
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    // two beats per iteration; state row r sits in slot [(r - y_lo) % 2]
+    auto pair_general = [&](int t) {
+        {
+            const int y = t - 8;
+            beat(I0(), t, valid(t), valid(t - 1), valid(t - 2), valid(t - 3), valid(t - 4), valid(t - 5),
+                 valid(t - 6), valid(t - 7), valid(y), valid(y) && y >= y_out_lo && y < y_out_hi, ia0, ib0, ea0, eb0,
+                 fa0, fb0, na0, nb0, ma0, mb0);
+        }
+        if (t + 2 < y_hi) load_psi(t + 2, ia0, ib0);
+        if (t < y_hi) load_e(t, ea0, eb0);
+        if (valid(t - 4)) load_e(t - 4, fa0, fb0);
+        {
+            const int u = t + 1, y = u - 8;
+            beat(I1(), u, valid(u), valid(u - 1), valid(u - 2), valid(u - 3), valid(u - 4), valid(u - 5),
+                 valid(u - 6), valid(u - 7), valid(y), valid(y) && y >= y_out_lo && y < y_out_hi, ia1, ib1, ea1, eb1,
+                 fa1, fb1, na1, nb1, ma1, mb1);
+        }
+        if (t + 3 < y_hi) load_psi(t + 3, ia1, ib1);
+        if (t + 1 < y_hi) load_e(t + 1, ea1, eb1);
+        if (valid(t - 3)) load_e(t - 3, fa1, fb1);
+    };
+    int t = y_lo;
+    if (t < y_hi) load_psi(t, ia0, ib0);
+    if (t + 1 < y_hi) load_psi(t + 1, ia1, ib1);
+    const int t_end = y_hi + 8;
+    // the pipelines fill
+    for (; t < t_end && t < y_lo + 8; t += 2) pair_general(t);
+    // steady state (rows t-8 .. t+1 in the segment, every emitted row before the
+    // end of the output range): no conditional memory operation
+    for (; t + 1 < y_hi; t += 2) {
+        beat(I0(), t, true, true, true, true, true, true, true, true, true, true, ia0, ib0, ea0, eb0, fa0, fb0, na0,
+             nb0, ma0, mb0);
+        load_psi_any(t + 2, ia0, ib0);
+        load_e(t, ea0, eb0);
+        load_e(t - 4, fa0, fb0);
+        beat(I1(), t + 1, true, true, true, true, true, true, true, true, true, true, ia1, ib1, ea1, eb1, fa1, fb1,
+             na1, nb1, ma1, mb1);
+        load_psi_any(t + 3, ia1, ib1);
+        load_e(t + 1, ea1, eb1);
+        load_e(t - 3, fa1, fb1);
+    }
+    // the pipelines drain
+    for (; t < t_end; t += 2) pair_general(t);
+}
+
+template <typename T, int VEC, bool PH>
+__global__ void __launch_bounds__(256, 2) pair2_kernel(PairParams<T> p) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
+    if (wave >= p.n_waves) return;
+    pair2_wave<T, VEC, PH>(p, wave, lane);
 }
 
 // ---------------------------------------------------------------------------

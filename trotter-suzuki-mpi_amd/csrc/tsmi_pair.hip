@@ -7,7 +7,9 @@
 // table and raw potential, their ping-pong buffers flipped together -- plus the
 // Rabi and inter-species constants and the imaginary-time norm targets, which
 // live on the device because the joint renormalisation changes them every step.
-// A step of both components is one pair_kernel launch (pair_kernel.hpp).
+// A step of both components is one pair_kernel launch (pair_kernel.hpp); in FP32
+// real time on closed or periodic lattices tsmi_pair_evolve runs two steps per
+// launch (pair2_kernel) on the components' 8-wide halo layout.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,7 +28,7 @@ using namespace tsmi;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kVecPair32 = 2;   // FP32 columns per lane of pair_kernel (FP64: 2)
+constexpr int kVecPair32 = 2;   // FP32 columns per lane of pair_kernel and pair2_kernel (FP64: 2)
 constexpr int kRabiBlocksMax = 1024;
 
 #define TSMI_HIPP(call)                                                                              \
@@ -44,6 +46,9 @@ struct tsmi_pair {
     double dt = 0, c_ab = 0, omega_r = 0, omega_i = 0, g_ab = 0, tot_norm = 0;
     int W = 0, H = 0, pitch = 0;
     int n_strips = 0, seg_rows = 0, n_segs = 0, n_waves = 0, rabi_blocks = 0;
+    // two steps per pass (pair2_kernel) over the whole allocation, Wd x Hd
+    bool fused = false, halo8 = false;
+    int Wd = 0, Hd = 0, n_strips2 = 0, seg_rows2 = 0, n_segs2 = 0, n_waves2 = 0;
     int ix0 = 0, ix1 = 0, iy0 = 0, iy1 = 0;
     // device scalars: divisor pairs [2][2] (a pending pair is read while the
     // next one is written), norm targets [2]
@@ -211,11 +216,82 @@ int pair_step(tsmi_pair *p, double var, bool rotate) {
     return ph ? pair_step_form<double, true>(p, var, rotate) : pair_step_form<double, false>(p, var, rotate);
 }
 
-// the periodic exchange of both components' current tiles
+// the periodic exchange of both components' current tiles (on the 8-wide halo
+// layout the full wrap, whose inner 4 cells are the reference's exchange)
 int exchange_both(tsmi_pair *p) {
-    for (int c = 0; c < 2; ++c)
-        if (int st = ctx_exchange(p->c[c], p->c[c]->psi_re[p->c[c]->cur], p->c[c]->psi_im[p->c[c]->cur])) return st;
+    for (int c = 0; c < 2; ++c) {
+        tsmi_ctx *x = p->c[c];
+        if (int st = ctx_exchange(x, x->psi_base[x->cur][0], x->psi_base[x->cur][1])) return st;
+    }
     return TSMI_OK;
+}
+
+// Two real-time FP32 steps of both components in one pass over the allocations;
+// the Rabi rotation by var1 ends the first, by var2 the second.  Phase form of
+// E only: the E-form instance needs more than 256 VGPRs (it spills).
+int pair_step2(tsmi_pair *p, double var1, double var2, bool rotate) {
+    constexpr bool PH = true;
+    using T = float;
+    PairParams<T> q;
+    std::memset(&q, 0, sizeof(q));
+    const int cur = p->c[0]->cur, nb = 1 - cur;
+    for (int i = 0; i < 2; ++i) {
+        tsmi_ctx *c = p->c[i];
+        q.in_re[i] = (const T *)c->psi_base[cur][0];
+        q.in_im[i] = (const T *)c->psi_base[cur][1];
+        q.out_re[i] = (T *)c->psi_base[nb][0];
+        q.out_im[i] = (T *)c->psi_base[nb][1];
+        q.e_re[i] = (const T *)(c->halo8 ? c->e8[0] : c->e_base[0]);
+        q.e_im[i] = (const T *)(c->halo8 ? c->e8[1] : c->e_base[1]);
+        q.ph[i] = (const T *)(c->halo8 ? c->ph8 : c->ph_base);
+        q.mag[i] = c->corr_abs;
+        q.mH[i] = (T)c->mH; q.tH[i] = (T)c->tH; q.mV[i] = (T)c->mV; q.tV[i] = (T)c->tV;
+        q.unH[i] = (T)c->unH; q.unV[i] = (T)c->unV;
+        q.c_self[i] = (T)c->c_nl_s;
+    }
+    q.W = p->Wd; q.H = p->Hd; q.pitch = p->pitch;
+    q.n_strips = p->n_strips2; q.n_waves = p->n_waves2; q.seg_rows = p->seg_rows2;
+    q.c_ab = (T)p->c_ab;
+    if (rotate) {
+        rabi_coefs(p, var1, &q.cc, &q.cs_r, &q.cs_i);
+        rabi_coefs(p, var2, &q.cc2, &q.cs_r2, &q.cs_i2);
+        q.rabi = 1;
+    }
+    const int blocks = (p->n_waves2 + kBlock / kWave - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL((pair2_kernel<T, kVecPair32, PH>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
+    TSMI_HIPP(hipGetLastError());
+    p->c[0]->cur = nb;
+    p->c[1]->cur = nb;
+    return TSMI_OK;
+}
+
+// Solver::evolve(n) with two steps per pass (src/solver.cpp:186-230), scheduled
+// as the single-component evolve_steps does: on closed axes the pairs replace
+// single steps from the first; on periodic axes the first step runs alone on
+// the reference tile, every pass starts from a freshly wrapped 8-wide halo, and
+// the last step runs alone, leaving the reference's unrefreshed halo.  The last
+// step's rotation is by 1/2 (src/solver.cpp:222-225).
+int evolve_fused(tsmi_pair *p, int n, bool rabi) {
+    auto var_of = [n](int i) { return i == n - 1 ? 0.5 : 1.0; };
+    int st = TSMI_OK, i = 0;
+    if (!p->halo8) {
+        for (; i + 1 < n && !st; i += 2) st = pair_step2(p, var_of(i), var_of(i + 1), rabi);
+        if (!st && i < n) st = pair_step(p, var_of(i), rabi);
+        return st;
+    }
+    if (n >= 3) {
+        st = pair_step(p, 1.0, rabi);
+        if (!st) st = exchange_both(p);
+        for (i = 1; n - i >= 3 && !st; i += 2) {
+            st = pair_step2(p, 1.0, 1.0, rabi);
+            if (!st) st = exchange_both(p);
+        }
+    }
+    for (; i < n && !st; ++i) {
+        st = pair_step(p, var_of(i), rabi);
+        if (!st && i != n - 1) st = exchange_both(p);
+    }
+    return st;
 }
 
 // Divides both current tiles by their pending divisors, if any.
@@ -260,6 +336,15 @@ int tsmi_pair_create(const tsmi_pair_desc *pd, tsmi_pair **out) {
                                          "time direction, precision and tile_rows");
     if (pd->a.coupling_lhy != 0.0 || pd->b.coupling_lhy != 0.0)
         return set_error(TSMI_E_UNSUPPORTED, "tsmi_pair_create: a two-component system has no LHY term");
+    // two steps per pass: FP32 real time, both axes closed or both periodic
+    const tsmi_desc &da = pd->a;
+    const bool fusable = da.precision == 32 && !da.imag_time &&
+                         ((da.periodic_x && da.periodic_y) || (!da.periodic_x && !da.periodic_y));
+    if (da.sweeps < 0 || da.sweeps > 2) return set_error(TSMI_E_INVALID, "tsmi_pair_create: sweeps must be 0, 1 or 2");
+    if (da.sweeps == 2 && !fusable)
+        return set_error(TSMI_E_UNSUPPORTED, "tsmi_pair_create: two steps per pass need FP32, real time and both "
+                                             "axes closed or both periodic");
+    const bool fused = fusable && da.sweeps != 1;
     tsmi_pair *p = new tsmi_pair();
     auto bail = [&](int code) {
         free_pair(p);
@@ -268,7 +353,7 @@ int tsmi_pair_create(const tsmi_pair_desc *pd, tsmi_pair **out) {
     };
     for (int i = 0; i < 2; ++i) {
         tsmi_desc d = i == 0 ? pd->a : pd->b;
-        d.sweeps = 1;
+        d.sweeps = fused ? 2 : 1;   // 2: the 8-wide halo layout on periodic axes, E wrapped over it
         if (int st = tsmi_create(&d, &p->c[i])) return bail(st);
     }
     if (int st = tsmi_set_stream(p->c[1], p->c[0]->stream)) return bail(st);
@@ -306,6 +391,34 @@ int tsmi_pair_create(const tsmi_pair_desc *pd, tsmi_pair **out) {
     p->n_segs = (p->H + p->seg_rows - 1) / p->seg_rows;
     p->n_waves = p->n_strips * p->n_segs;
     p->rabi_blocks = (int)std::min<long>(kRabiBlocksMax, ((long)p->W * p->H + kRabiThreads - 1) / kRabiThreads);
+    p->fused = fused;
+    p->halo8 = c0->halo8;
+    p->Wd = c0->Wd; p->Hd = c0->Hd;
+    if (fused) {
+        // strips of kWave*vec - 16 output columns over the allocation; segments
+        // sized to one round of resident waves of pair2_kernel, at least 16 rows
+        const int sw2 = kWave * kVecPair32 - 4 * kApron;
+        p->n_strips2 = (p->Wd + sw2 - 1) / sw2;
+        int seg2 = d.tile_rows;
+        if (seg2 <= 0) {
+            seg2 = std::max(16, (p->Hd + 1) / 2);
+            int dev = 0, cus = 0, per_cu = 0;
+            if (hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pair2_kernel<float, kVecPair32, true>, kBlock,
+                                                             0) == hipSuccess &&
+                cus > 0 && per_cu > 0) {
+                const long resident = (long)cus * per_cu * (kBlock / kWave);
+                const long segs = std::max<long>(1, resident / p->n_strips2);
+                seg2 = (int)std::max<long>(16, (p->Hd + segs - 1) / segs);
+            }
+        }
+        seg2 = std::min(seg2, max_seg_rows(p->pitch));
+        seg2 = (seg2 + 1) / 2 * 2;   // segments start on even rows
+        p->seg_rows2 = std::min(seg2, p->Hd + (p->Hd & 1));
+        p->n_segs2 = (p->Hd + p->seg_rows2 - 1) / p->seg_rows2;
+        p->n_waves2 = p->n_strips2 * p->n_segs2;
+    }
     // norm targets at kernel construction (src/solver.cpp:153-159, src/cpukernel.cpp:321-325)
     const double nt[6] = {1.0, 1.0, 1.0, 1.0, pd->a.norm_target, pd->b.norm_target};
     p->tot_norm = nt[4] + nt[5];
@@ -355,6 +468,8 @@ int tsmi_pair_evolve(tsmi_pair *p, int32_t n_steps, int32_t initial_half) {
     int st = TSMI_OK;
     // src/solver.cpp:173-177
     if (initial_half && rabi) st = rabi_pass(p, 0.5, true, false);
+    // two steps per pass where both E tables have the phase form
+    if (p->fused && p->c[0]->ph_mode && p->c[1]->ph_mode) return st ? st : evolve_fused(p, n_steps, rabi);
     for (int i = 0; i < n_steps && !st; ++i) {
         const bool last = i == n_steps - 1;
         const double var = last ? 0.5 : 1.0;   // src/solver.cpp:222-225

@@ -20,11 +20,11 @@ ENERGY_KEYS = ("total", "kinetic_a", "kinetic_b", "potential_a", "potential_b", 
                "rabi", "norm_a", "norm_b", "norm")
 
 
-def _desc(pw, which: int, precision: int, norm_target: float, tile_rows: int) -> _lib.Desc:
+def _desc(pw, which: int, precision: int, norm_target: float, tile_rows: int, sweeps: int = 0) -> _lib.Desc:
     lat = pw.lattice
     mass, g = (pw.mass_a, pw.g_a) if which == 0 else (pw.mass_b, pw.g_b)
     return _lib.Desc(lat.dim_x, lat.dim_y, int(lat.periodic_x), int(lat.periodic_y), lat.halo_x, lat.halo_y,
-                     int(pw.imag_time), int(precision), int(tile_rows), 1, float(pw.dt), float(mass), float(g), 0.0,
+                     int(pw.imag_time), int(precision), int(tile_rows), int(sweeps), float(pw.dt), float(mass), float(g), 0.0,
                      lat.delta_x, lat.delta_y, lat.length_x, lat.length_y, float(norm_target))
 
 
@@ -35,14 +35,15 @@ def squared_norm(lat, psi: np.ndarray) -> float:
 
 
 class HipPair:
-    def __init__(self, pw, precision: int = 64, tile_rows: int = 0, e_planes=None, norm_targets=None):
+    def __init__(self, pw, precision: int = 64, tile_rows: int = 0, e_planes=None, norm_targets=None,
+                 sweeps: int = 0):
         L = _lib.load()
         self._L = L
         self.pw = pw
         if norm_targets is None:
             norm_targets = (squared_norm(pw.lattice, pw.psi_a), squared_norm(pw.lattice, pw.psi_b))
-        d = _lib.PairDesc(_desc(pw, 0, precision, norm_targets[0], tile_rows),
-                          _desc(pw, 1, precision, norm_targets[1], tile_rows), float(pw.g_ab), float(pw.omega_r),
+        d = _lib.PairDesc(_desc(pw, 0, precision, norm_targets[0], tile_rows, sweeps),
+                          _desc(pw, 1, precision, norm_targets[1], tile_rows, sweeps), float(pw.g_ab), float(pw.omega_r),
                           float(pw.omega_i))
         h = C.c_void_p()
         _lib.check(L.tsmi_pair_create(C.byref(d), C.byref(h)), "tsmi_pair_create")
