@@ -102,6 +102,28 @@ def test_pair_two_steps_per_pass_fp32(oracle, case, tile_rows):
     assert rel_l2(a2, ra) < 2e-5 and rel_l2(b2, rb) < 2e-5
 
 
+def test_pair_single_steps_on_two_step_layout(oracle):
+    """tsmi_pair_rabi / tsmi_pair_step driven one step at a time (the reference
+    solver's protocol) on a pair created for two steps per pass -- periodic
+    axes, so the 8-wide halo layout and its full-wrap exchange -- give the state
+    of one-step-per-pass evolve."""
+    from tsmi import _lib
+    from tsmi.pair import HipPair
+    pw = make(CASES[4])
+    n = pw.calls[0]
+    e = oracle.pair_exp_potentials(pw)
+    with HipPair(pw, precision=32, e_planes=e, sweeps=1) as h:
+        h.evolve(n)
+        a1, b1 = h.state(0), h.state(1)
+    with HipPair(pw, precision=32, e_planes=e, sweeps=2) as h:
+        _lib.check(h._L.tsmi_pair_rabi(h._h, 0.5, 0), "tsmi_pair_rabi")
+        for i in range(n):
+            last = i == n - 1
+            _lib.check(h._L.tsmi_pair_step(h._h, 0 if last else 1, 0.5 if last else 1.0), "tsmi_pair_step")
+        a2, b2 = h.state(0), h.state(1)
+    assert rel_l2(a2, a1) < 1e-6 and rel_l2(b2, b1) < 1e-6
+
+
 @pytest.mark.parametrize("precision,case", [(64, CASES[1]), (32, CASES[2]), (32, CASES[8])],
                          ids=["fp64", "fp32_mixed_axes", "fp32_imag"])
 def test_pair_two_steps_per_pass_refused(precision, case):
