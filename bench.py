@@ -12,6 +12,17 @@ Prints one JSON line: value = site-updates/s of the FP32 step (inner sites x
 steps / time), with the roofline of the step kernel, the FP64 figures, and the
 reference CPU kernel timed on this host's cores on a bounded sample.
 
+roofline.achieved is the HBM bandwidth the dominant kernel would need for the
+bytes it must move: per launch, the state read once and written once and E read
+once per inner site (FP32 with E as one phase plane: 20 B; FP64: 40 B), whatever
+the number of steps the launch advances; frac = achieved / 8 TB/s.  The
+single-sweep figure (24 B per site and step, times the steps per launch) is
+reported beside it as single_sweep_equivalent_GBs: it exceeds the peak when a
+launch advances two steps.
+
+The CPU baseline runs first, before anything in this process touches the GPU
+(a process that has initialised the GPU must not start another program).
+
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -20,7 +31,6 @@ import argparse
 import json
 import os
 import sys
-import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -28,11 +38,17 @@ sys.path.insert(0, os.path.join(ROOT, "trotter-suzuki-mpi_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
-BYTES_PER_SU = {32: 24, 64: 48}   # psi read + psi write + E read per inner site (real time)
-# What the kernel reads and writes per inner site and step when E is a pure phase
-# (the case here): E as one phase plane instead of two planes
-KERNEL_BYTES_PER_SU = {32: 20, 64: 40}
+BYTES_PER_SU = {32: 24, 64: 48}   # psi read + psi write + E read (two planes) per inner site and step
+# What a launch must move per inner site when E is a pure phase (the case here):
+# the state read and written once, E read as one phase plane -- per launch,
+# whether the launch advances one step or two
+KERNEL_BYTES_PER_SITE = {32: 20, 64: 40}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_8192_traffic.json")
+# the sources the step kernels are built from: a traffic record measured on
+# other sources is not reported
+KERNEL_SOURCES = ("trotter-suzuki-mpi_amd/csrc/step_kernel.hpp", "trotter-suzuki-mpi_amd/csrc/fused_kernel.hpp",
+                  "trotter-suzuki-mpi_amd/csrc/tsmi_hip.hip")
+ACCURACY_STEPS = 1000
 
 
 def log(msg: str) -> None:
@@ -49,6 +65,9 @@ def parse():
     ap.add_argument("--precision", choices=["both", "32", "64"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=12, help="timed steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP32-vs-FP64 comparison over 1000 steps (when --steps is not 1000)")
+    ap.add_argument("--no-api", action="store_true", help="skip timing Solver::evolve through the C++ API")
     ap.add_argument("--no-studies", action="store_true",
                     help="skip the precision study, the batch, adjoint and two-component measurements")
     ap.add_argument("--only-study", choices=["precision_study", "batch", "adjoint", "pair"],
@@ -107,18 +126,45 @@ def time_precision(prec, lat, psi_re, psi_im, e_re, e_im, v, params, steps, warm
     return ms, out_re, out_im, norm, geo, kt
 
 
+def host_cpu() -> dict:
+    """This host's CPU: model, logical CPUs, the ones this process may use, and
+    whether it has AVX-512 (x86-64-v4)."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "avx512": v4}
+
+
 def cpu_baseline(lat, psi, v, params, steps):
-    """The reference CPU kernel (oracle/_ref) on this host, on the full lattice."""
+    """The reference CPU kernel (oracle/_ref) on this host, on the full lattice:
+    its OpenMP CPUBlock built from its sources with its release flags, for
+    AVX-512 (x86-64-v4) where the CPU has it, else for x86-64-v3.  Threads: the
+    CPUs this process may use, at most 16 (the host share of one GPU)."""
     from oracle import pyoracle
     from tsmi.workloads import Workload
-    threads = min(16, len(os.sched_getaffinity(0)))
+    cpu = host_cpu()
+    threads = min(16, cpu["affinity_cpus"])
     wl = Workload("config2", lat, psi, v, params["dt"], steps, g=params["g"], mass=params["mass"])
-    sample = f"{lat.dim_x}x{lat.dim_y} periodic GPE (the benchmark lattice), 1 warm-up + {steps} timed steps"
-    if pyoracle.ref_available("fast"):
-        r = pyoracle.time_reference(wl, steps=steps, n_warmup=1, kind="fast", threads=threads)
+    sample = f"{lat.dim_x}x{lat.dim_y} periodic GPE (the benchmark lattice and state), 1 warm-up + {steps} timed steps"
+    host = {"cpu_model": cpu["model"], "host_logical_cpus": cpu["logical_cpus"],
+            "process_cpus": cpu["affinity_cpus"]}
+    kind = "fast4" if cpu["avx512"] and pyoracle.ref_available("fast4") else "fast"
+    if pyoracle.ref_available(kind):
+        r = pyoracle.time_reference(wl, steps=steps, n_warmup=1, kind=kind, threads=threads)
         su = lat.dim_x * lat.dim_y * r["steps"] / r["seconds"]
+        march = "x86-64-v4" if kind == "fast4" else "x86-64-v3"
         return {"value": su, "unit": "site-updates/s", "cores": r["threads"], "kind": "reference",
-                "sample": "reference CPUBlock built from its sources (-Ofast -march=x86-64-v3 -fopenmp), " + sample}
+                "sample": f"reference CPUBlock built from its sources (-Ofast -march={march} -fopenmp), " + sample,
+                **host}
     # the oracle restatement (OpenMP over rows) when the reference build is absent
     os.environ["OMP_NUM_THREADS"] = str(threads)
     import numpy as np
@@ -128,9 +174,60 @@ def cpu_baseline(lat, psi, v, params, steps):
     pyoracle.lib().ts_oracle_evolve(d, 1, pyoracle._p(re), pyoracle._p(im), pyoracle._p(er), pyoracle._p(ei))
     t = time.perf_counter()
     pyoracle.lib().ts_oracle_evolve(d, steps, pyoracle._p(re), pyoracle._p(im), pyoracle._p(er), pyoracle._p(ei))
-    s = time.perf_counter() - t
-    return {"value": lat.dim_x * lat.dim_y * steps / s, "unit": "site-updates/s", "cores": threads, "kind": "port",
-            "sample": "oracle restatement (OpenMP over rows), " + sample}
+    sec = time.perf_counter() - t
+    return {"value": lat.dim_x * lat.dim_y * steps / sec, "unit": "site-updates/s", "cores": threads, "kind": "port",
+            "sample": "oracle restatement (OpenMP over rows), " + sample, **host}
+
+
+def sources_sha16() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(prec: int, n: int):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    record, if it was measured on these kernel sources at this size."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, "no counter record"
+    with open(TRAFFIC_FILE) as f:
+        rec = json.load(f)
+    if rec.get("source_sha16") != sources_sha16():
+        return None, "counter record measured on other kernel sources"
+    tr = rec.get(f"fp{prec}")
+    if not tr or tr.get("n") != n:
+        return None, "no counter record at this size"
+    return tr.get("bytes_per_launch"), TRAFFIC_FILE[len(ROOT) + 1:]
+
+
+def api_leg(lat, psi_h, v_h, params, prec: int, steps: int, warmup: int = 4):
+    """Solver::evolve(steps) through the C++ API (Lattice2D, State, Potential,
+    Hamiltonian, Solver with kernel type "hip-fp32"/"hip"), in this process."""
+    import ctypes as C
+    import numpy as np
+    from tsmi import _lib
+    _lib.load()   # the HIP library first, with PyTorch's runtime (see tsmi._lib)
+    path = os.path.join(ROOT, "trotter-suzuki-mpi_amd", "lib", "libtsmi_api_timing.so")
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    L.tsmi_api_time_evolve.argtypes = [C.c_int32, C.c_double, C.c_int32, dp, dp, dp, C.c_double, C.c_double,
+                                       C.c_double, C.c_int32, C.c_int32, C.c_int32, dp]
+    L.tsmi_api_last_error.restype = C.c_char_p
+    re, im = np.ascontiguousarray(psi_h.real), np.ascontiguousarray(psi_h.imag)
+    v = np.ascontiguousarray(v_h)
+    out = np.zeros(4)
+    st = L.tsmi_api_time_evolve(lat.dim_x, lat.length_x, 1, re.ctypes.data_as(dp), im.ctypes.data_as(dp),
+                                v.ctypes.data_as(dp), params["dt"], params["mass"], params["g"], prec, warmup, steps,
+                                out.ctypes.data_as(dp))
+    if st != 0:
+        raise RuntimeError(L.tsmi_api_last_error().decode())
+    per_step = (out[1] - out[2]) / steps
+    return {"route": f"Solver(..., \"{'hip-fp32' if prec == 32 else 'hip'}\").evolve({steps}) through the C++ API",
+            "ms_per_step": per_step * 1e3, "site_updates_per_s": lat.dim_x * lat.dim_y / per_step,
+            "evolve_s": out[1], "final_get_sample_s": out[2], "kernel_build_s": out[0], "norm": out[3]}
 
 
 def precision_study(steps: int = 10000):
@@ -276,6 +373,52 @@ def pair_study(n: int = 4096, steps: int = 200):
     return out
 
 
+def evolve_precision(prec, lat, psi_re, psi_im, e_re, e_im, params, steps):
+    """steps steps from the initial state; returns the final planes and the
+    squared norms before and after (FP64 sums on the device)."""
+    import torch
+    from tsmi.hip import HipKernel
+    k = HipKernel(lat, delta_t=params["dt"], mass=params["mass"], coupling=params["g"], precision=prec)
+    try:
+        k.set_stream(torch.cuda.current_stream().cuda_stream)
+        k.set_exp_potential_device(e_re.data_ptr(), e_im.data_ptr())
+        k.set_state_device(psi_re.data_ptr(), psi_im.data_ptr())
+        n0 = k.squared_norm()
+        k.evolve(steps)
+        out_re, out_im = torch.empty_like(psi_re), torch.empty_like(psi_im)
+        k.get_state_device(out_re.data_ptr(), out_im.data_ptr())
+        torch.cuda.synchronize()
+        return out_re, out_im, n0, k.squared_norm()
+    finally:
+        k.close()
+
+
+def rel_l2_inner(lat, a, b):
+    import torch
+    iy, ix = lat.inner_y, lat.inner_x
+    x = torch.complex(*[t[iy, ix] for t in a])
+    y = torch.complex(*[t[iy, ix] for t in b])
+    return float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y))
+
+
+def roofline_of(r, prec, sites, n):
+    """The dominant kernel's roofline: the bytes a launch must move per inner
+    site (state in, state out, E as one phase plane) over its mean duration."""
+    t = r["kernel_us"] * 1e-6
+    need = KERNEL_BYTES_PER_SITE[prec] * sites
+    achieved = need / t / 1e9
+    traffic, src = measured_traffic(prec, n)
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": r["kernel"], "kernel_us_per_launch": r["kernel_us"], "launches": r["launches"],
+            "steps_per_launch": r["sweeps"], "bytes_per_site_per_launch": KERNEL_BYTES_PER_SITE[prec],
+            "bytes_per_launch": need, "e_form": "phase plane (E of a real potential as one plane)",
+            "traffic_source": src,
+            "traffic_over_required": traffic / need if traffic else None,
+            "traffic_rate_GBs": traffic / t / 1e9 if traffic else None,
+            "single_sweep_equivalent_GBs": r["sweeps"] * BYTES_PER_SU[prec] * sites / t / 1e9}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -284,22 +427,43 @@ def main():
               "(multi-GPU sharding is a design note, not part of this build); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
-    if not torch.cuda.is_available():
-        print("bench.py needs a GPU", file=sys.stderr)
-        sys.exit(2)
-    if args.calibrate_copy:
-        calibrate_copy()
-        return
-    if args.only_study:
+    if args.calibrate_copy or args.only_study:
+        if not torch.cuda.is_available():
+            print("bench.py needs a GPU", file=sys.stderr)
+            sys.exit(2)
+        if args.calibrate_copy:
+            calibrate_copy()
+            return
         fn = {"precision_study": precision_study, "batch": batch_study, "adjoint": adjoint_study,
               "pair": pair_study}[args.only_study]
         print(json.dumps({args.only_study: fn()}), flush=True)
         return
     from tsmi import workloads_torch as WT
 
+    # The CPU baseline first: it starts the reference program, which this process
+    # may do only before it has touched the GPU.
+    cpu = None
+    psi_h = v_h = None
+    if not args.no_cpu_baseline:
+        log(f"building the {args.n}^2 vortex lattice on the host")
+        lat, psi_c, v_c, params = WT.config2(n=args.n, device="cpu")
+        psi_h, v_h = psi_c.numpy(), v_c.numpy()
+        del psi_c, v_c
+        log(f"CPU baseline: reference kernel, {args.cpu_steps} steps of the full lattice")
+        cpu = cpu_baseline(lat, psi_h, v_h, params, args.cpu_steps)
+        log(f"CPU baseline: {cpu['value']:.4g} SU/s on {cpu['cores']} threads ({cpu['kind']}, {cpu['cpu_model']}, "
+            f"{cpu['host_logical_cpus']} logical CPUs)")
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
     torch.cuda.set_device(0)
-    log(f"building the {args.n}^2 vortex lattice on the device")
-    lat, psi, v, params = WT.config2(n=args.n, device="cuda")
+    if psi_h is not None:
+        psi = torch.from_numpy(psi_h).to("cuda")
+        v = torch.from_numpy(v_h).to("cuda")
+    else:
+        log(f"building the {args.n}^2 vortex lattice on the device")
+        lat, psi, v, params = WT.config2(n=args.n, device="cuda")
     psi_re, psi_im = psi.real.contiguous(), psi.imag.contiguous()
     e_re, e_im = WT.exp_potential(v, params["dt"], False)
     sites = lat.dim_x * lat.dim_y
@@ -316,40 +480,46 @@ def main():
         fused = kt["fused_launches"] > 0
         launches = kt["fused_launches"] if fused else kt["single_launches"]
         kms = (kt["fused_ms"] if fused else kt["single_ms"]) / max(1, launches)
-        sweeps = 2 if fused else 1
-        ach = sweeps * BYTES_PER_SU[prec] * sites / (kms * 1e-3) / 1e9
-        results[prec] = dict(ms=ms, per_step=per_step, su=su, achieved=ach, norm=norm, geo=geo, out=(o_re, o_im),
+        results[prec] = dict(ms=ms, per_step=per_step, su=su, norm=norm, geo=geo, out=(o_re, o_im),
                              kernel="step2_kernel" if fused else "step_kernel", kernel_us=kms * 1e3,
-                             launches=launches, sweeps=sweeps, kt=kt)
-        log(f"FP{prec}: {per_step * 1e3:.1f} us/step, {su:.4g} SU/s; {results[prec]['kernel']} "
-            f"{kms * 1e3:.1f} us/launch x {launches} ({sweeps} steps each): {ach:.0f} GB/s algorithmic; "
-            f"norm {norm:.15f}")
+                             launches=launches, sweeps=2 if fused else 1, kt=kt)
+        rf = roofline_of(results[prec], prec, sites, args.n)
+        results[prec]["roofline"] = rf
+        log(f"FP{prec}: {per_step * 1e3:.1f} us/step, {su:.4g} SU/s; {rf['kernel']} {kms * 1e3:.1f} us/launch x "
+            f"{launches} ({rf['steps_per_launch']} steps each): {rf['achieved']:.0f} GB/s of the bytes a launch "
+            f"must move = {rf['frac']:.3f} of HBM peak; norm {norm:.15f}")
 
-    rel = None
-    if 32 in results and 64 in results:
-        iy, ix = lat.inner_y, lat.inner_x
-        a32 = torch.complex(*[t[iy, ix] for t in results[32]["out"]])
-        a64 = torch.complex(*[t[iy, ix] for t in results[64]["out"]])
-        rel = float(torch.linalg.vector_norm(a32 - a64) / torch.linalg.vector_norm(a64))
-        log(f"FP32 vs FP64 after {args.steps} steps: relative L2 {rel:.3e}")
+    accuracy = None
+    if 32 in results and 64 in results and not args.no_accuracy:
+        if args.steps == ACCURACY_STEPS:
+            outs = {p: (results[p]["out"], None, results[p]["norm"]) for p in (32, 64)}
+            n0 = {}
+        else:
+            log(f"accuracy: FP32 and FP64, {ACCURACY_STEPS} steps each from the same state")
+            outs, n0 = {}, {}
+            for p in (32, 64):
+                o_re, o_im, a, b = evolve_precision(p, lat, psi_re, psi_im, e_re, e_im, params, ACCURACY_STEPS)
+                outs[p] = ((o_re, o_im), a, b)
+                n0[p] = a
+        rel = rel_l2_inner(lat, outs[32][0], outs[64][0])
+        accuracy = {"steps": ACCURACY_STEPS, "fp32_rel_l2_vs_fp64": rel,
+                    "norm_drift_fp32": abs(outs[32][2] - (n0.get(32) or 1.0)),
+                    "norm_drift_fp64": abs(outs[64][2] - (n0.get(64) or 1.0))}
+        log(f"FP32 vs FP64 after {ACCURACY_STEPS} steps: relative L2 {rel:.3e}; norm drift FP32 "
+            f"{accuracy['norm_drift_fp32']:.2e}, FP64 {accuracy['norm_drift_fp64']:.2e}")
+
+    api = None
+    if not args.no_api and 32 in results:
+        log("C++ API: Solver::evolve through HipKernel (hip-fp32)")
+        if psi_h is None:
+            psi_h, v_h = psi.cpu().numpy(), v.cpu().numpy()
+        torch.cuda.synchronize()
+        api = api_leg(lat, psi_h, v_h, params, 32, max(args.steps, 100))
+        log(f"C++ API: {api['ms_per_step'] * 1e3:.1f} us/step ({api['site_updates_per_s']:.4g} SU/s), "
+            f"final get_sample {api['final_get_sample_s'] * 1e3:.1f} ms")
 
     main_prec = precs[0]
     r = results[main_prec]
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            tr = json.load(f).get(f"fp{main_prec}")
-        if tr and tr.get("n") == args.n:
-            traffic = tr.get("bytes_per_launch")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        log(f"CPU baseline: reference kernel, {args.cpu_steps} steps of the full lattice")
-        psi_h = psi.cpu().numpy()
-        v_h = v.cpu().numpy()
-        cpu = cpu_baseline(lat, psi_h, v_h, params, args.cpu_steps)
-        log(f"CPU baseline: {cpu['value']:.4g} SU/s on {cpu['cores']} threads ({cpu['kind']})")
-
     line = {
         "metric": "site-updates/s per Trotter step & % HBM peak; FP32 vs FP64 at fixed L2 error",
         "value": r["su"],
@@ -362,31 +532,23 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": f"f{main_prec}",
-        "data": "synthetic (seeded 16-vortex Thomas-Fermi state built on the device)",
+        "data": "synthetic (seeded 16-vortex Thomas-Fermi state)",
         "config": {"workload": f"real-time GPE vortex dynamics, {lat.dim_x}x{lat.dim_y} periodic, side 64, "
                                "g=50, trap 0.01, dt=5e-4",
                    "lattice": [lat.dim_x, lat.dim_y], "tile": [lat.W, lat.H], "geometry": r["geo"],
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": r["kernel"], "kernel_us_per_launch": r["kernel_us"], "launches": r["launches"],
-                     "bytes_per_site_update": BYTES_PER_SU[main_prec], "sweeps_per_launch": r["sweeps"],
-                     "algorithmic_bytes_per_launch": r["sweeps"] * BYTES_PER_SU[main_prec] * sites,
-                     "e_form": "phase plane (one plane of E's two)",
-                     "kernel_bytes_per_site_update": KERNEL_BYTES_PER_SU[main_prec],
-                     "kernel_bytes_achieved": r["achieved"] * KERNEL_BYTES_PER_SU[main_prec] / BYTES_PER_SU[main_prec]},
+        "roofline": r["roofline"],
         "cpu_baseline": cpu,
     }
     if 64 in results and main_prec != 64:
         r64 = results[64]
-        line["fp64"] = {"value": r64["su"], "ms_per_step": r64["per_step"],
-                        "roofline": {"bound": "hbm", "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
-                                     "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
-                                     "kernel": r64["kernel"], "kernel_us_per_launch": r64["kernel_us"],
-                                     "bytes_per_site_update": BYTES_PER_SU[64], "sweeps_per_launch": r64["sweeps"]}}
+        line["fp64"] = {"value": r64["su"], "ms_per_step": r64["per_step"], "roofline": r64["roofline"]}
         line["fp32_over_fp64"] = r["su"] / r64["su"]
-        line["fp32_rel_l2_vs_fp64"] = rel
-        line["norm_drift"] = {"fp32": abs(results[32]["norm"] - 1.0), "fp64": abs(r64["norm"] - 1.0)}
+    if accuracy:
+        line["fp32_rel_l2_vs_fp64"] = accuracy["fp32_rel_l2_vs_fp64"]
+        line["accuracy"] = accuracy
+    if api:
+        line["api"] = api
     if not args.no_studies:
         for name, fn in (("precision_study", precision_study), ("batch", batch_study), ("adjoint", adjoint_study),
                          ("pair", pair_study)):

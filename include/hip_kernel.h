@@ -13,6 +13,14 @@
  * Supported: one component, Cartesian coordinates, angular velocity 0, real or
  * imaginary time, closed or periodic axes, FP64 or FP32 bulk.
  *
+ * HipKernel queues the Solver's per-step calls (run_kernel, with or without the
+ * halo exchange that start_halo_exchange asks for) and issues the queue as one
+ * tsmi_run_steps call when something reads or changes the state: get_sample,
+ * calculate_squared_norm, update_potential, the energies, or a step that follows
+ * one without an exchange.  A run of steps therefore reaches the library's
+ * two-step pass (two steps per HBM round trip where that is exact) with the
+ * reference's halo contents at every call boundary.
+ *
  * HipPairKernel is the sibling of the two-state CPUBlock constructor
  * (src/cpukernel.cpp:274-392) for a two-component system, with the same
  * arguments.  The Solver drives a step as run_kernel + wait_for_completion for
@@ -39,7 +47,7 @@ public:
               double *external_pot_imag, double delta_t, double norm, bool imag_time, int precision = 64);
     ~HipKernel();
 
-    void run_kernel();                 ///< one step; the halo exchange if start_halo_exchange() preceded it
+    void run_kernel();                 ///< queues one step; the halo exchange if start_halo_exchange() preceded it
     void run_kernel_on_halo();         ///< no-op: the step kernel covers the whole tile
     void wait_for_completion();        ///< no-op: the renormalisation is part of the step
     void get_sample(size_t dest_stride, size_t x, size_t y, size_t width, size_t height,
@@ -59,13 +67,16 @@ public:
     void set_potential(const double *v);   ///< raw V on the tile, for energy()
     void energy(double out[6]) const;  ///< total, kinetic, potential, intra, LHY, squared norm
     void expected_values(double out[10]) const;
-    tsmi_ctx *context() const { return ctx_; }
+    void flush() const;                ///< issues the queued steps
+    tsmi_ctx *context() const { return ctx_; }   ///< flush() before using it directly
 
 private:
     tsmi_ctx *ctx_;
     int precision_;
     int width_, height_;
     bool exchange_next_;
+    mutable int queued_;               ///< steps requested and not yet issued
+    mutable bool queued_exchange_;     ///< whether the last of them is followed by the exchange
 };
 
 class HipPairKernel : public ITrotterKernel {

@@ -111,6 +111,13 @@ int tsmi_set_potential_exp_device(tsmi_ctx *ctx, const double *v);
  * wait_for_completion (:551-570).  Asynchronous on the context stream. */
 int tsmi_evolve(tsmi_ctx *ctx, int32_t n_steps);
 
+/* n_steps consecutive run_kernel calls as Solver::evolve issues them
+ * (src/solver.cpp:201-210): each step followed by the periodic halo exchange,
+ * the last one only when exchange_last != 0.  tsmi_evolve(n) is
+ * tsmi_run_steps(n, 0); a kernel object that queues the Solver's per-step calls
+ * issues them with this, so they reach the two-step pass.  Asynchronous. */
+int tsmi_run_steps(tsmi_ctx *ctx, int32_t n_steps, int32_t exchange_last);
+
 /* One step, followed by the periodic halo exchange when exchange != 0:
  * run_kernel_on_halo + run_kernel with start/finish_halo_exchange
  * (src/cpukernel.cpp:413-524, :691-747) and, in imaginary time,
@@ -286,6 +293,17 @@ int tsmi_synchronize(tsmi_ctx *ctx);
 /* Tile geometry chosen for the context: out[0..5] = W, H, pitch (elements),
  * strips, segments, rows per segment.  No reference counterpart (reporting). */
 int tsmi_geometry(const tsmi_ctx *ctx, int32_t out[6]);
+
+/* The geometry tsmi_create would choose for a descriptor, without creating a
+ * context or touching device memory (with no device to query, the segment
+ * heights assume an unknown occupancy).  out[0..11] = W, H, pitch (elements),
+ * strips, segments and rows per segment of the one-step pass, steps per pass
+ * of tsmi_evolve (1 or 2), strips, segments and rows per segment of the
+ * two-step pass (0 when it is not used), and the allocation's width and height
+ * (the 8-wide halo layout of a doubly periodic lattice at two steps per pass).
+ * Every segment plus its apron rows spans less than 2^31 bytes of an FP64
+ * plane.  No reference counterpart (planning and reporting). */
+int tsmi_plan_geometry(const tsmi_desc *desc, int32_t out[12]);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char *tsmi_last_error(void);

@@ -15,7 +15,14 @@
 // protocol run per entry of the case's call list, against the reference
 // two-state Solver.
 //
-//   dropin_check case.bin [precision]   ->  one JSON line
+//   dropin_check case.bin [precision [calls [update_at]]]   ->  one JSON line
+//
+// calls ("7,5"): single component, one Solver::evolve call per entry instead of
+// the case's step count, on both sides.  update_at: before that step (counted
+// over all calls) the kernel gets ITrotterKernel::update_potential with the same
+// table, as Solver::evolve does when Potential::update reports a change
+// (src/solver.cpp:187-192).  The line reports the number of two-step passes the
+// kernel ran (fused_launches).
 #include <trottersuzuki.h>
 #include <hip_kernel.h>
 
@@ -24,6 +31,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 static void die(const char *m) { std::fprintf(stderr, "dropin_check: %s\n", m); std::exit(2); }
@@ -121,7 +129,18 @@ int main(int argc, char **argv) {
     if (std::fread(re.data(), 8, n, f) != n || std::fread(im.data(), 8, n, f) != n || std::fread(v.data(), 8, n, f) != n)
         die("short data");
     std::fclose(f);
-    const int steps = h[6];
+    std::vector<int> calls;
+    if (argc > 3) {
+        for (const char *p = argv[3]; *p;) {
+            calls.push_back(std::atoi(p));
+            p = std::strchr(p, ',');
+            if (!p) break;
+            ++p;
+        }
+    } else {
+        calls.push_back(h[6]);
+    }
+    const int update_at = argc > 4 ? std::atoi(argv[4]) : -1;
     const bool imag = h[7] != 0;
     const double dt = par[2];
 
@@ -131,7 +150,7 @@ int main(int argc, char **argv) {
     Potential pot(&grid, v.data());
     Hamiltonian ham(&grid, &pot, par[3], par[4], par[5]);
     Solver solver(&grid, &cstate, &ham, dt, "cpu");
-    solver.evolve(steps, imag);
+    for (int c : calls) solver.evolve(c, imag);
 
     // the same evolution through HipKernel, as Solver::evolve drives a kernel
     State hstate(&grid, 0, re.data(), im.data());
@@ -145,18 +164,27 @@ int main(int argc, char **argv) {
             eim[(size_t)y * grid.dim_x + x] = e.imag();
         }
     const double norm = imag ? hstate.get_squared_norm() : 0.;
-    ITrotterKernel *k = new HipKernel(&grid, &hstate, &ham, ere.data(), eim.data(), dt, norm, imag, precision);
-    for (int i = 0; i < steps; ++i) {
-        k->run_kernel_on_halo();
-        if (i != steps - 1) k->start_halo_exchange();
-        k->run_kernel();
-        if (i != steps - 1) k->finish_halo_exchange();
-        k->wait_for_completion();
-        k->cpy_first_positive_to_first_negative();
-    }
+    HipKernel *hk = new HipKernel(&grid, &hstate, &ham, ere.data(), eim.data(), dt, norm, imag, precision);
+    ITrotterKernel *k = hk;
+    if (tsmi_kernel_timing(hk->context(), 1) != TSMI_OK) die("kernel timing");
+    int done = 0;
     std::vector<double> ore(n), oim(n);
-    k->get_sample(grid.dim_x, 0, 0, grid.dim_x, grid.dim_y, ore.data(), oim.data());
+    for (int steps : calls) {
+        for (int i = 0; i < steps; ++i, ++done) {
+            if (done == update_at) k->update_potential(ere.data(), eim.data(), 0);
+            k->run_kernel_on_halo();
+            if (i != steps - 1) k->start_halo_exchange();
+            k->run_kernel();
+            if (i != steps - 1) k->finish_halo_exchange();
+            k->wait_for_completion();
+            k->cpy_first_positive_to_first_negative();
+        }
+        k->get_sample(grid.dim_x, 0, 0, grid.dim_x, grid.dim_y, ore.data(), oim.data());
+    }
     const double kn = k->calculate_squared_norm();
+    double kt[4] = {0, 0, 0, 0};
+    hk->flush();
+    if (tsmi_kernel_time(hk->context(), kt) != TSMI_OK) die("kernel time");
     const std::string name = k->get_name();
     delete k;
 
@@ -166,7 +194,8 @@ int main(int argc, char **argv) {
         num += dr * dr + di * di;
         den += cre[i] * cre[i] + cim[i] * cim[i];
     }
-    std::printf("{\"kernel\": \"%s\", \"rel_l2\": %.6e, \"kernel_norm\": %.17g, \"solver_norm\": %.17g}\n",
-                name.c_str(), std::sqrt(num / den), kn, solver.get_squared_norm());
+    std::printf("{\"kernel\": \"%s\", \"rel_l2\": %.6e, \"kernel_norm\": %.17g, \"solver_norm\": %.17g, "
+                "\"fused_launches\": %d, \"single_launches\": %d}\n",
+                name.c_str(), std::sqrt(num / den), kn, solver.get_squared_norm(), (int)kt[1], (int)kt[3]);
     return 0;
 }

@@ -1,0 +1,241 @@
+"""The benchmark workloads at their full sizes on the GPU.
+
+* 8192^2 periodic real-time GPE vortex lattice (g = 50, dt = 5e-4): the FP64 HIP
+  path on its default route (two steps per HBM pass, 8-wide halo layout) against
+  the oracle after 6 steps over the whole tile; the FP32 bulk against the oracle
+  after 6 steps; FP32 against FP64 after the workload's 1000 steps (relative L2
+  and energy), with the FP64 norm conserved.
+* 2048^2 closed imaginary-time GPE ground state (g = 100, dt = 1e-4): FP64
+  against the oracle after 20 steps over the whole tile; FP32 against FP64 after
+  all 10000 steps (relative L2, energy), the renormalised norm, and the energy
+  decreasing along the run (imaginary time lowers it monotonically).
+* 512 lattices of seeded sides 64..1024, closed or periodic: the FP64 batch,
+  graph-replayed, against the oracle on one lattice of every (side, boundary)
+  class after 20 steps; the FP32 batch after the autotuner and 500
+  graph-captured steps against single FP32 contexts and FP64 on the same
+  lattices, and every lattice's norm.
+
+The oracle is the checker; the HIP path runs through the C ABI.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tsmi import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[configs] {msg}", file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def inner(lat, psi):
+    return psi[lat.inner_y, lat.inner_x]
+
+
+# ---------------------------------------------------------------------------
+# 8192^2 real-time vortex lattice
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def vortex():
+    """The 8192^2 workload, built on the device and copied to the host, so the
+    oracle and the HIP path start from the same tile."""
+    import torch
+    from tsmi import workloads_torch as WT
+    lat, psi, v, params = WT.config2(n=8192, device="cuda")
+    wl = W.Workload("config2", lat, psi.cpu().numpy(), v.cpu().numpy(), params["dt"], 1000, g=params["g"])
+    del psi, v
+    torch.cuda.empty_cache()
+    return wl
+
+
+def _kernel(wl, precision, e):
+    from tsmi.hip import kernel_for
+    return kernel_for(wl, precision=precision, e_planes=e)
+
+
+def test_vortex_8192_fused_route_matches_oracle(vortex, oracle):
+    wl = vortex
+    e = oracle.exp_potential(wl.v, wl.dt, False)
+    say("8192^2 vortex: oracle, 6 steps")
+    ref = oracle.evolve(wl, steps=6)
+    for prec, tol in ((64, 1e-12), (32, 2e-5)):
+        say(f"8192^2 vortex: FP{prec} HIP, 6 steps on the default route")
+        k = _kernel(wl, prec, e)
+        try:
+            k.kernel_timing(True)
+            k.evolve(6)
+            kt = k.kernel_time()
+            psi = k.sample()
+        finally:
+            k.close()
+        # the first and last step alone, the four between as two fused passes
+        assert kt["fused_launches"] == 2 and kt["single_launches"] == 2
+        r = rel_l2(psi, ref)
+        say(f"8192^2 vortex: FP{prec} vs oracle after 6 steps, whole tile: {r:.3e}")
+        assert r < tol
+
+
+def test_vortex_8192_fp32_vs_fp64_1000_steps(vortex, oracle):
+    wl = vortex
+    e = oracle.exp_potential(wl.v, wl.dt, False)
+    out = {}
+    for prec in (64, 32):
+        say(f"8192^2 vortex: FP{prec}, 1000 steps")
+        k = _kernel(wl, prec, e)
+        try:
+            n0 = k.squared_norm()
+            e0 = k.energy()
+            k.evolve(1000)
+            out[prec] = dict(psi=k.sample(), n0=n0, n=k.squared_norm(), e0=e0, e=k.energy())
+        finally:
+            k.close()
+    lat = wl.lattice
+    r = rel_l2(inner(lat, out[32]["psi"]), inner(lat, out[64]["psi"]))
+    de = abs(out[32]["e"][0] - out[64]["e"][0]) / abs(out[64]["e"][0])
+    d64 = abs(out[64]["n"] - out[64]["n0"])
+    d32 = abs(out[32]["n"] - out[32]["n0"])
+    say(f"8192^2 vortex after 1000 steps: FP32 vs FP64 rel L2 {r:.3e}, energy {de:.3e}; "
+        f"norm drift FP64 {d64:.2e}, FP32 {d32:.2e}")
+    assert r <= 1e-4
+    assert de <= 1e-4
+    assert d64 <= 1e-12
+    assert d32 <= 1e-4
+    # real time conserves the energy of the GPE functional up to the splitting error
+    assert abs(out[64]["e"][0] - out[64]["e0"][0]) / abs(out[64]["e0"][0]) < 1e-3
+
+
+# ---------------------------------------------------------------------------
+# 2048^2 imaginary-time ground state
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def ground():
+    return W.config1()
+
+
+def test_ground_state_2048_fp64_matches_oracle(ground, oracle):
+    wl = ground
+    d = oracle.make_desc(wl)
+    e = oracle.exp_potential(wl.v, wl.dt, True)
+    say("2048^2 imaginary time: oracle, 20 steps")
+    ref = oracle.evolve(wl, steps=20)
+    from tsmi.hip import kernel_for
+    say("2048^2 imaginary time: FP64 HIP, 20 steps")
+    k = kernel_for(wl, precision=64, norm_target=d.norm_target, e_planes=e)
+    try:
+        k.evolve(20)
+        psi = k.sample()
+        en = k.energy()
+    finally:
+        k.close()
+    r = rel_l2(psi, ref)
+    say(f"2048^2 imaginary time: FP64 vs oracle after 20 steps, whole tile: {r:.3e}")
+    assert r < 1e-12
+    np.testing.assert_allclose(en, oracle.energy(wl, ref), rtol=1e-12, atol=1e-14)
+
+
+def test_ground_state_2048_fp32_vs_fp64_10000_steps(ground, oracle):
+    wl = ground
+    d = oracle.make_desc(wl)
+    e = oracle.exp_potential(wl.v, wl.dt, True)
+    from tsmi.hip import kernel_for
+    out = {}
+    for prec in (64, 32):
+        say(f"2048^2 imaginary time: FP{prec}, 10000 steps in two calls")
+        k = kernel_for(wl, precision=prec, norm_target=d.norm_target, e_planes=e)
+        try:
+            e0 = k.energy()[0]
+            k.evolve(5000)
+            e1 = k.energy()[0]
+            k.evolve(5000)
+            out[prec] = dict(psi=k.sample(), n=k.squared_norm(), energy=k.energy()[0], e0=e0, e1=e1)
+        finally:
+            k.close()
+    lat = wl.lattice
+    r = rel_l2(inner(lat, out[32]["psi"]), inner(lat, out[64]["psi"]))
+    de = abs(out[32]["energy"] - out[64]["energy"]) / abs(out[64]["energy"])
+    say(f"2048^2 imaginary time after 10000 steps: FP32 vs FP64 rel L2 {r:.3e}, energy {de:.3e}; "
+        f"E(0, 5000, 10000) FP64 = {out[64]['e0']:.9f}, {out[64]['e1']:.9f}, {out[64]['energy']:.9f}")
+    assert r <= 1e-5
+    assert de <= 1e-6
+    # renormalised every step to the initial squared norm
+    assert abs(out[64]["n"] - d.norm_target) <= 1e-12 * d.norm_target
+    assert abs(out[32]["n"] - d.norm_target) <= 1e-6 * d.norm_target
+    for p in (64, 32):
+        assert out[p]["e0"] > out[p]["e1"] > out[p]["energy"]
+
+
+# ---------------------------------------------------------------------------
+# 512 lattices in one batch
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def batch512():
+    wls = W.config3()
+    # one lattice of every (side, boundary) class
+    picks = {}
+    for i, w in enumerate(wls):
+        picks.setdefault((w.lattice.dim_x, w.lattice.periodic_x), i)
+    return wls, sorted(picks.values())
+
+
+def test_batch_512_fp64_graph_replays_match_oracle(batch512, oracle):
+    from tsmi.batch import HipBatch
+    wls, picks = batch512
+    say(f"512 lattices: FP64 batch, 4 x evolve(5) (graphs recorded, then replayed); {len(picks)} checked")
+    with HipBatch(wls, precision=64) as b:
+        for _ in range(4):
+            b.evolve(5)
+        got = {i: b.state(i) for i in picks}
+    for i in picks:
+        wl = wls[i]
+        psi = wl.psi
+        for _ in range(4):
+            psi = oracle.evolve(wl, steps=5, psi=psi)
+        r = rel_l2(got[i], psi)
+        say(f"512 lattices: lattice {i} ({wl.lattice.dim_x}^2, periodic {wl.lattice.periodic_x}) "
+            f"vs oracle after 20 steps: {r:.3e}")
+        assert r < 1e-12, i
+
+
+def test_batch_512_fp32_500_steps_matches_single_contexts(batch512):
+    from tsmi.batch import HipBatch
+    from tsmi.hip import kernel_for
+    wls, picks = batch512
+    say("512 lattices: FP32 batch, autotuner, 500 graph-captured steps")
+    with HipBatch(wls, precision=32) as b:
+        b.tune(steps=3)
+        b.evolve(500)
+        got = {i: b.state(i) for i in range(len(wls))}
+    # every lattice keeps its norm (real time)
+    worst = 0.0
+    for i, w in enumerate(wls):
+        n0 = W.squared_norm(w.lattice, w.psi)
+        worst = max(worst, abs(W.squared_norm(w.lattice, got[i]) - n0) / n0)
+    say(f"512 lattices: largest relative norm drift after 500 FP32 steps {worst:.2e}")
+    assert worst < 1e-4
+    for i in picks:
+        wl = wls[i]
+        ref = {}
+        for prec in (32, 64):
+            k = kernel_for(wl, precision=prec)
+            try:
+                k.evolve(500)
+                ref[prec] = k.sample()
+            finally:
+                k.close()
+        lat = wl.lattice
+        r32 = rel_l2(inner(lat, got[i]), inner(lat, ref[32]))
+        r64 = rel_l2(inner(lat, got[i]), inner(lat, ref[64]))
+        say(f"512 lattices: lattice {i} ({lat.dim_x}^2, periodic {lat.periodic_x}) after 500 steps: "
+            f"vs single FP32 {r32:.2e}, vs FP64 {r64:.2e}")
+        assert r32 < 1e-5, i
+        assert r64 < 1e-4, i

@@ -135,7 +135,7 @@ void free_batch(tsmi_batch *b) {
 }
 
 void set_rows(Lattice &L, int seg) {
-    seg = std::max(2, std::min(seg, std::min(L.H, max_seg_rows(L.pitch))));
+    seg = std::max(2, std::min(seg, std::min(L.H, max_seg_rows(L.pitch, 2 * kApron))));
     seg = (seg + 1) / 2 * 2;   // segments start on even rows
     L.seg_rows = std::min(seg, L.H + (L.H & 1));
     L.n_segs = (L.H + L.seg_rows - 1) / L.seg_rows;
@@ -144,7 +144,7 @@ void set_rows(Lattice &L, int seg) {
 
 void set_rows2(Lattice &L, int seg) {
     if (!L.fusable) return;
-    seg = std::max(2, std::min(seg, std::min(L.Hd, max_seg_rows(L.pitch))));
+    seg = std::max(2, std::min(seg, std::min(L.Hd, max_seg_rows(L.pitch, 4 * kApron))));
     seg = (seg + 1) / 2 * 2;
     L.seg_rows2 = std::min(seg, L.Hd + (L.Hd & 1));
     L.n_segs2 = (L.Hd + L.seg_rows2 - 1) / L.seg_rows2;

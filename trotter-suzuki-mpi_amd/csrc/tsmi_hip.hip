@@ -53,21 +53,38 @@ int check_desc(const tsmi_desc &d, const char *who) {
     if ((d.periodic_x && d.dim_x < 2 * d.halo_x) || (d.periodic_y && d.dim_y < 2 * d.halo_y))
         return fail(TSMI_E_INVALID, w + ": a periodic axis needs at least 2*halo points");
     if (d.tile_rows < 0) return fail(TSMI_E_INVALID, w + ": tile_rows must be >= 0");
-    const long long W = (long long)d.dim_x + (d.periodic_x ? 2 * d.halo_x : 0);
-    const long long H = (long long)d.dim_y + (d.periodic_y ? 2 * d.halo_y : 0);
+    // the widest allocation: the 8-wide halo layout of two steps per pass
+    const long long W = (long long)d.dim_x + (d.periodic_x ? 4 * d.halo_x : 0);
+    const long long H = (long long)d.dim_y + (d.periodic_y ? 4 * d.halo_y : 0);
     const long long pitch = (W + 63) / 64 * 64;
-    // a wave addresses its segment with 32-bit byte offsets: one FP64 row of
-    // (apron + 2) rows must fit well inside 2^31 bytes
-    if (pitch * 8 * 16 >= (1LL << 31) || H > (1 << 30))
+    // a wave addresses its segment with 32-bit byte offsets: the smallest
+    // segment of a two-step pass (2 rows and 16 apron rows) of FP64 must stay
+    // below 2^31 bytes
+    if (pitch * 8 * (2 + 4 * kApron) >= (1LL << 31) || H > (1 << 30))
         return fail(TSMI_E_INVALID, w + ": lattice too large");
     return TSMI_OK;
 }
 
-// Largest even segment height whose rows plus the 4-row aprons keep a wave's
-// byte offsets below 2^31 (FP64 gradient planes included).
-int max_seg_rows(long long pitch) {
-    const long long rows = ((1LL << 31) - 1) / (pitch * 8) - 2 * kApron;
+int max_seg_rows(long long pitch, int apron_rows) {
+    const long long rows = ((1LL << 31) - 1) / (pitch * 8) - apron_rows;
     return (int)std::max<long long>(2, std::min<long long>(rows, 1 << 30) / 2 * 2);
+}
+
+int plan_seg_rows(int H, int strips, long resident, long simds, int floor_rows, bool small12, int tile_rows,
+                  long long pitch, int apron_rows) {
+    int seg = tile_rows;
+    if (seg <= 0) {
+        seg = std::max<int>(floor_rows, (H + 1) / 2);
+        if (resident > 0) {
+            const long segs = std::max<long>(1, resident / std::max(1, strips));
+            seg = (int)std::max<long>(floor_rows, (H + segs - 1) / segs);
+            if (small12 && (long)strips * ((H + seg - 1) / seg) < simds)
+                seg = (int)std::max<long>(12, (H + segs - 1) / segs);
+        }
+    }
+    seg = std::min(seg, max_seg_rows(pitch, apron_rows));
+    seg = (seg + 1) / 2 * 2;
+    return std::min(seg, H);
 }
 
 }  // namespace tsmi
@@ -317,33 +334,44 @@ int exchange_halo(tsmi_ctx *c, void *base_re, void *base_im) {
 // Segments of the fused pass sized to one round of resident waves of the
 // instance that runs (its registers, hence its occupancy, depend on the scaled
 // form and on the E form); recomputed when either changes.
+// Resident waves of a kernel on the current device at its occupancy (0 when
+// there is no device to ask) and the device's SIMD count.
+template <typename K>
+long resident_waves(K kernel, long *simds) {
+    int dev = 0, cus = 0, per_cu = 0;
+    *simds = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || cus <= 0 ||
+        per_cu <= 0) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    *simds = 4L * cus;
+    return (long)cus * per_cu * (kBlock / kWave);
+}
+
+// Geometry of the two-step pass over the allocation (Wd x Hd): strips of
+// kWave*VEC - 16 output columns, segments of one round of resident waves, at
+// least 16 rows -- 12 where 16-row segments leave SIMDs without a wave: a
+// wave's row latency then bounds the pass (measured, 1024^2 closed, 2000 steps:
+// FP32 33.1 -> 30.3 ms, FP64 38.1 -> 34.9 ms).
+void plan_fused(int Wd, int Hd, long long pitch, int tile_rows, int vec, long resident, long simds, int *strips,
+                int *rows, int *segs) {
+    const int sw2 = kWave * vec - 4 * kApron;
+    *strips = (Wd + sw2 - 1) / sw2;
+    *rows = plan_seg_rows(Hd, *strips, resident, simds, 16, true, tile_rows, pitch, 4 * kApron);
+    *segs = (Hd + *rows - 1) / *rows;
+}
+
 template <typename T, int VEC, bool LHY, int MODE, bool PH>
 int size_fused(tsmi_ctx *c) {
     const int key = (MODE + 1) * 2 + (PH ? 1 : 0);
     if (c->geo2_key == key) return TSMI_OK;
-    const int sw2 = kWave * VEC - 4 * kApron;
-    c->n_strips2 = (c->Wd + sw2 - 1) / sw2;
-    int seg2 = std::max<int>(16, (c->Hd + 1) / 2);
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step2_kernel<T, VEC, LHY, MODE, PH>, kBlock, 0) ==
-            hipSuccess &&
-        cus > 0 && per_cu > 0) {
-        const long resident = (long)cus * per_cu * (kBlock / kWave);
-        const long segs = std::max<long>(1, resident / c->n_strips2);
-        seg2 = (int)std::max<long>(16, (c->Hd + segs - 1) / segs);
-        // where 16-row segments leave SIMDs without a wave, a wave's row latency
-        // bounds the pass and 12-row segments are faster (measured, 1024^2 closed,
-        // 2000 steps: FP32 33.1 -> 30.3 ms, FP64 38.1 -> 34.9 ms)
-        if ((long)c->n_strips2 * ((c->Hd + seg2 - 1) / seg2) < 4L * cus)
-            seg2 = (int)std::max<long>(12, (c->Hd + segs - 1) / segs);
-    }
-    if (c->d.tile_rows > 0) seg2 = c->d.tile_rows;
-    seg2 = std::min(seg2, max_seg_rows(c->pitch));
-    seg2 = (seg2 + 1) / 2 * 2;
-    c->seg_rows2 = std::min<int>(seg2, c->Hd);
-    c->n_segs2 = (c->Hd + c->seg_rows2 - 1) / c->seg_rows2;
+    long simds = 0;
+    const long resident = resident_waves(step2_kernel<T, VEC, LHY, MODE, PH>, &simds);
+    plan_fused(c->Wd, c->Hd, c->pitch, c->d.tile_rows, VEC, resident, simds, &c->n_strips2, &c->seg_rows2,
+               &c->n_segs2);
     c->n_waves2 = c->n_strips2 * c->n_segs2;
     c->geo2_key = key;
     return TSMI_OK;
@@ -456,6 +484,29 @@ hipError_t occupancy_blocks(int precision, bool imag, bool lhy, int *blocks) {
     return lhy ? occupancy_of<double, false, 2, true>(blocks) : occupancy_of<double, false, 2, false>(blocks);
 }
 
+// Rows per segment of the one-step pass.  One round of resident wavefronts: as
+// many row segments per strip as the device holds at this kernel's occupancy,
+// so no partial second round idles most of the GPU; on small lattices at least
+// 24 rows, below which the 4-row aprons and the per-wave partial sums cost more
+// than the extra waves bring -- unless 24-row segments leave SIMDs without a
+// wave, where a wave's row latency bounds the step and 12-row segments are
+// faster (measured, imaginary time: 1024^2 FP32 28.9 -> 22.5 us, FP64 30.3 ->
+// 23.2; 2048^2 FP32 33.7 -> 31.3; FP64 at 2048^2, one wave per SIMD at 24 rows,
+// 50.0 vs 52.3 with 12).
+int plan_single(const tsmi_desc &d, int H, long long pitch, int strips) {
+    long resident = 0, simds = 0;
+    int dev = 0, cus = 0, blocks_per_cu = 0;
+    if (d.tile_rows <= 0 && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        occupancy_blocks(d.precision, d.imag_time != 0, d.coupling_lhy != 0.0, &blocks_per_cu) == hipSuccess &&
+        cus > 0 && blocks_per_cu > 0) {
+        resident = (long)cus * blocks_per_cu * (kBlock / kWave);
+        simds = 4L * cus;
+    }
+    (void)hipGetLastError();
+    return plan_seg_rows(H, strips, resident, simds, 24, true, d.tile_rows, pitch, 2 * kApron);
+}
+
 template <typename T, bool IMAG, int VEC, bool LHY, bool PH>
 int step_form(tsmi_ctx *c, int exchange) {
     if constexpr (IMAG) {
@@ -491,21 +542,23 @@ int exchange_planes(tsmi_ctx *c, void *re, void *im) {
 }
 
 // Solver::evolve(n) for one component (src/solver.cpp:186-230): n steps, the
-// halo exchange after all but the last.  With two steps per pass, on closed
-// axes the pairs simply replace single steps; on periodic axes (8-wide halo
-// layout) the first step runs alone on the reference tile (its halo is whatever
-// the caller or the previous call left, as in the reference), every pass starts
-// from a freshly wrapped halo, and the last step runs alone so that the halo it
-// leaves is the reference's unrefreshed one.
-int evolve_steps(tsmi_ctx *c, int n) {
+// halo exchange after all but the last (after the last too when exchange_last).
+// With two steps per pass, on closed axes the pairs simply replace single
+// steps; on periodic axes (8-wide halo layout) the first step runs alone on the
+// reference tile (its halo is whatever the caller or the previous call left, as
+// in the reference), every pass starts from a freshly wrapped halo, and the last
+// step runs alone so that the halo it leaves is the reference's unrefreshed one
+// (or the refreshed one, when exchange_last).
+int evolve_steps(tsmi_ctx *c, int n, bool exchange_last = false) {
     int st = TSMI_OK;
+    const int last_ex = exchange_last ? 1 : 0;
     if (c->sweeps != 2 || n < 2 || (c->halo8 && n < 3)) {
-        for (int i = 0; i < n && !st; ++i) st = step_once(c, i != n - 1 ? 1 : 0);
+        for (int i = 0; i < n && !st; ++i) st = step_once(c, i != n - 1 ? 1 : last_ex);
         return st;
     }
     if (!c->halo8) {
         for (int i = 0; i + 1 < n && !st; i += 2) st = step2_once(c);
-        if (!st && (n & 1)) st = step_once(c, 0);
+        if (!st && (n & 1)) st = step_once(c, last_ex);
         return st;
     }
     st = step_once(c, 1);
@@ -515,7 +568,7 @@ int evolve_steps(tsmi_ctx *c, int n) {
         if (!st) st = exchange_planes(c, c->psi_base[c->cur][0], c->psi_base[c->cur][1]);
     }
     if (!st && m == 1) st = step_once(c, 1);
-    if (!st) st = step_once(c, 0);
+    if (!st) st = step_once(c, last_ex);
     return st;
 }
 
@@ -694,34 +747,7 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     c->vec = c->esz == 4 ? (d.imag_time ? kVec32I : kVec32) : 2;
     const int sw = kWave * c->vec - 2 * kApron;
     c->n_strips = (c->W + sw - 1) / sw;
-    int seg = d.tile_rows;
-    if (seg <= 0) {
-        // Size the launch to one round of resident wavefronts: as many row
-        // segments per strip as the device holds at this kernel's occupancy, so
-        // no partial second round idles most of the GPU; on small lattices at
-        // least 24 rows, below which the 4-row aprons and the per-wave partial
-        // sums cost more than the extra waves bring -- unless 24-row segments
-        // leave SIMDs without a wave, where a wave's row latency bounds the step
-        // and 12-row segments are faster (measured, imaginary time: 1024^2 FP32
-        // 28.9 -> 22.5 us, FP64 30.3 -> 23.2; 2048^2 FP32 33.7 -> 31.3; FP64 at
-        // 2048^2, one wave per SIMD at 24 rows, 50.0 vs 52.3 with 12).
-        seg = std::max<int>(24, (int)((H + 1) / 2));
-        int dev = 0, cus = 0, blocks_per_cu = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            occupancy_blocks(d.precision, d.imag_time != 0, d.coupling_lhy != 0.0, &blocks_per_cu) == hipSuccess &&
-            cus > 0 && blocks_per_cu > 0) {
-            const long resident = (long)cus * blocks_per_cu * (kBlock / kWave);
-            const long segs = std::max<long>(1, resident / c->n_strips);
-            seg = (int)std::max<long>(24, (H + segs - 1) / segs);
-            const long simds = 4L * cus;
-            if ((long)c->n_strips * ((H + seg - 1) / seg) < simds)
-                seg = (int)std::max<long>(12, (H + segs - 1) / segs);
-        }
-    }
-    seg = std::min(seg, max_seg_rows(pitch));
-    seg = (seg + 1) / 2 * 2;   // segments start on even rows (the kernel's pair patterns assume it)
-    c->seg_rows = std::min<int>(seg, c->H);
+    c->seg_rows = plan_single(d, c->H, pitch, c->n_strips);
     c->n_segs = (c->H + c->seg_rows - 1) / c->seg_rows;
     c->n_waves = c->n_strips * c->n_segs;
     c->ix0 = d.periodic_x ? d.halo_x : 0; c->ix1 = c->W - c->ix0;
@@ -920,6 +946,14 @@ int tsmi_evolve(tsmi_ctx *c, int32_t n) {
     if (n < 0) return fail(TSMI_E_INVALID, "tsmi_evolve: negative step count");
     if (!c->has_state || !c->has_e) return fail(TSMI_E_INVALID, "tsmi_evolve: state and exp potential must be set");
     return evolve_steps(c, n);
+}
+
+int tsmi_run_steps(tsmi_ctx *c, int32_t n, int32_t exchange_last) {
+    if (c) c->ck_valid = false;
+    if (!c) return fail(TSMI_E_INVALID, "tsmi_run_steps: null context");
+    if (n < 0) return fail(TSMI_E_INVALID, "tsmi_run_steps: negative step count");
+    if (!c->has_state || !c->has_e) return fail(TSMI_E_INVALID, "tsmi_run_steps: state and exp potential must be set");
+    return evolve_steps(c, n, exchange_last != 0);
 }
 
 int tsmi_step(tsmi_ctx *c, int32_t exchange) {
@@ -1122,6 +1156,56 @@ int tsmi_geometry(const tsmi_ctx *c, int32_t out[6]) {
     if (!c || !out) return fail(TSMI_E_INVALID, "tsmi_geometry: null argument");
     out[0] = c->W; out[1] = c->H; out[2] = c->pitch;
     out[3] = c->n_strips; out[4] = c->n_segs; out[5] = c->seg_rows;
+    return TSMI_OK;
+}
+
+int tsmi_plan_geometry(const tsmi_desc *desc, int32_t out[12]) {
+    if (!desc || !out) return fail(TSMI_E_INVALID, "tsmi_plan_geometry: null argument");
+    const tsmi_desc &d = *desc;
+    if (int st = check_desc(d, "tsmi_plan_geometry")) return st;
+    if (d.sweeps < 0 || d.sweeps > 2) return fail(TSMI_E_INVALID, "tsmi_plan_geometry: sweeps must be 0, 1 or 2");
+    const bool fusable = !d.imag_time && ((d.periodic_x && d.periodic_y) || (!d.periodic_x && !d.periodic_y));
+    if (d.sweeps == 2 && !fusable)
+        return fail(TSMI_E_UNSUPPORTED,
+                    "tsmi_plan_geometry: two steps per pass need real time and both axes closed or both periodic");
+    const int sweeps = (d.sweeps != 1 && fusable) ? 2 : 1;
+    const bool halo8 = sweeps == 2 && d.periodic_x && d.periodic_y;
+    const int W = d.dim_x + (d.periodic_x ? 2 * d.halo_x : 0), H = d.dim_y + (d.periodic_y ? 2 * d.halo_y : 0);
+    const int Wd = W + (halo8 ? 2 * d.halo_x : 0), Hd = H + (halo8 ? 2 * d.halo_y : 0);
+    const long long pitch = ((long long)Wd + 63) / 64 * 64;
+    const int esz = d.precision == 32 ? 4 : 8;
+    const int vec = esz == 4 ? (d.imag_time ? kVec32I : kVec32) : 2;
+    const int strips = (W + kWave * vec - 2 * kApron - 1) / (kWave * vec - 2 * kApron);
+    const int rows = plan_single(d, H, pitch, strips);
+    for (int i = 0; i < 12; ++i) out[i] = 0;
+    out[0] = W; out[1] = H; out[2] = (int)pitch;
+    out[3] = strips; out[4] = (H + rows - 1) / rows; out[5] = rows;
+    out[6] = sweeps; out[10] = Wd; out[11] = Hd;
+    if (sweeps == 2) {
+        // the instance the first pass of a table tabulated from a real
+        // potential launches: E as one phase plane, the frame's scaled form
+        const StepCoefs k = step_coefs(d, H, esz);
+        const bool lhy = d.coupling_lhy != 0.0;
+        long simds = 0, resident = 0;
+        if (esz == 4) {
+            if (k.mode == 0) resident = lhy ? resident_waves(step2_kernel<float, kVec32F, true, 0, true>, &simds)
+                                            : resident_waves(step2_kernel<float, kVec32F, false, 0, true>, &simds);
+            else if (k.mode == 1) resident = lhy ? resident_waves(step2_kernel<float, kVec32F, true, 1, true>, &simds)
+                                                 : resident_waves(step2_kernel<float, kVec32F, false, 1, true>, &simds);
+            else resident = lhy ? resident_waves(step2_kernel<float, kVec32F, true, -1, true>, &simds)
+                                : resident_waves(step2_kernel<float, kVec32F, false, -1, true>, &simds);
+        } else {
+            if (k.mode == 0) resident = lhy ? resident_waves(step2_kernel<double, 2, true, 0, true>, &simds)
+                                            : resident_waves(step2_kernel<double, 2, false, 0, true>, &simds);
+            else if (k.mode == 1) resident = lhy ? resident_waves(step2_kernel<double, 2, true, 1, true>, &simds)
+                                                 : resident_waves(step2_kernel<double, 2, false, 1, true>, &simds);
+            else resident = lhy ? resident_waves(step2_kernel<double, 2, true, -1, true>, &simds)
+                                : resident_waves(step2_kernel<double, 2, false, -1, true>, &simds);
+        }
+        int s2 = 0, r2 = 0, n2 = 0;
+        plan_fused(Wd, Hd, pitch, d.tile_rows, esz == 4 ? kVec32F : 2, resident, simds, &s2, &r2, &n2);
+        out[7] = s2; out[8] = n2; out[9] = r2;
+    }
     return TSMI_OK;
 }
 

@@ -385,7 +385,7 @@ int tsmi_pair_create(const tsmi_pair_desc *pd, tsmi_pair **out) {
             seg = (int)std::max<long>(24, (p->H + segs - 1) / segs);
         }
     }
-    seg = std::min(seg, max_seg_rows(p->pitch));
+    seg = std::min(seg, max_seg_rows(p->pitch, 2 * kApron));
     seg = (seg + 1) / 2 * 2;   // segments start on even rows
     p->seg_rows = std::min(seg, p->H + (p->H & 1));
     p->n_segs = (p->H + p->seg_rows - 1) / p->seg_rows;
@@ -413,7 +413,7 @@ int tsmi_pair_create(const tsmi_pair_desc *pd, tsmi_pair **out) {
                 seg2 = (int)std::max<long>(16, (p->Hd + segs - 1) / segs);
             }
         }
-        seg2 = std::min(seg2, max_seg_rows(p->pitch));
+        seg2 = std::min(seg2, max_seg_rows(p->pitch, 4 * kApron));
         seg2 = (seg2 + 1) / 2 * 2;   // segments start on even rows
         p->seg_rows2 = std::min(seg2, p->Hd + (p->Hd & 1));
         p->n_segs2 = (p->Hd + p->seg_rows2 - 1) / p->seg_rows2;

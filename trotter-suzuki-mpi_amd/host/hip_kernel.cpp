@@ -17,7 +17,8 @@ void check(int status, const char *what) {
 
 HipKernel::HipKernel(Lattice *grid, State *state, Hamiltonian *hamiltonian, double *external_pot_real,
                      double *external_pot_imag, double delta_t, double norm, bool imag_time, int precision)
-    : ctx_(nullptr), precision_(precision), width_(0), height_(0), exchange_next_(false) {
+    : ctx_(nullptr), precision_(precision), width_(0), height_(0), exchange_next_(false), queued_(0),
+      queued_exchange_(false) {
     if (grid->coordinate_system != "cartesian")
         throw std::runtime_error("The HIP kernel supports Cartesian coordinates only.");
     if (hamiltonian->angular_velocity != 0.)
@@ -64,9 +65,22 @@ HipKernel::~HipKernel() { tsmi_destroy(ctx_); }
 
 void HipKernel::run_kernel() {
     // one step; the exchange follows unless this is the last step of an evolve
-    // call (the Solver then skips start_halo_exchange, src/solver.cpp:203-209)
-    check(tsmi_step(ctx_, exchange_next_ ? 1 : 0), "HIP kernel");
+    // call (the Solver then skips start_halo_exchange, src/solver.cpp:203-209).
+    // Queued: a run of steps each followed by the exchange, the last one with or
+    // without it, is what tsmi_run_steps issues in one call.
+    if (queued_ > 0 && !queued_exchange_) flush();
+    ++queued_;
+    queued_exchange_ = exchange_next_;
     exchange_next_ = false;
+}
+
+void HipKernel::flush() const {
+    if (queued_ == 0) return;
+    const int n = queued_;
+    const bool ex = queued_exchange_;
+    queued_ = 0;
+    queued_exchange_ = false;
+    check(tsmi_run_steps(ctx_, n, ex ? 1 : 0), "HIP kernel");
 }
 
 void HipKernel::run_kernel_on_halo() {}
@@ -78,6 +92,7 @@ void HipKernel::start_halo_exchange() { exchange_next_ = true; }
 void HipKernel::get_sample(size_t dest_stride, size_t x, size_t y, size_t width, size_t height, double *dest_real,
                            double *dest_imag, double *dest_real2, double *dest_imag2) const {
     if (dest_real2 || dest_imag2) throw std::runtime_error("The HIP kernel evolves one component.");
+    flush();
     check(tsmi_get_sample(ctx_, dest_stride, x, y, width, height, dest_real, dest_imag), "HIP kernel");
 }
 
@@ -86,6 +101,7 @@ void HipKernel::normalization() { throw std::runtime_error("The HIP kernel evolv
 void HipKernel::rabi_coupling(double, double) { throw std::runtime_error("The HIP kernel evolves one component."); }
 
 double HipKernel::calculate_squared_norm(bool) const {
+    flush();
     double n = 0;
     check(tsmi_squared_norm(ctx_, &n), "HIP kernel");
     return n;
@@ -93,16 +109,26 @@ double HipKernel::calculate_squared_norm(bool) const {
 
 void HipKernel::update_potential(double *re, double *im, int which) {
     if (which != 0) throw std::runtime_error("The HIP kernel evolves one component.");
+    flush();
     check(tsmi_set_exp_potential(ctx_, re, im), "HIP kernel");
 }
 
-void HipKernel::evolve(int n_steps) { check(tsmi_evolve(ctx_, n_steps), "HIP kernel"); }
+void HipKernel::evolve(int n_steps) {
+    flush();
+    check(tsmi_evolve(ctx_, n_steps), "HIP kernel");
+}
 
 void HipKernel::set_potential(const double *v) { check(tsmi_set_potential(ctx_, v), "HIP kernel"); }
 
-void HipKernel::energy(double out[6]) const { check(tsmi_energy(ctx_, out), "HIP kernel"); }
+void HipKernel::energy(double out[6]) const {
+    flush();
+    check(tsmi_energy(ctx_, out), "HIP kernel");
+}
 
-void HipKernel::expected_values(double out[10]) const { check(tsmi_expected_values(ctx_, out), "HIP kernel"); }
+void HipKernel::expected_values(double out[10]) const {
+    flush();
+    check(tsmi_expected_values(ctx_, out), "HIP kernel");
+}
 
 // ---------------------------------------------------------------------------
 // HipPairKernel

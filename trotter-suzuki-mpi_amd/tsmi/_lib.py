@@ -19,11 +19,12 @@ TSMI_OK = 0
 EXPORTS = (
     "tsmi_create", "tsmi_destroy", "tsmi_set_stream", "tsmi_set_state", "tsmi_set_state_device",
     "tsmi_set_exp_potential", "tsmi_set_exp_potential_device", "tsmi_set_potential",
-    "tsmi_set_potential_device", "tsmi_set_potential_exp_device", "tsmi_evolve", "tsmi_step", "tsmi_evolve_checkpointed", "tsmi_adjoint",
+    "tsmi_set_potential_device", "tsmi_set_potential_exp_device", "tsmi_evolve", "tsmi_run_steps", "tsmi_step", "tsmi_evolve_checkpointed", "tsmi_adjoint",
     "tsmi_adjoint_device", "tsmi_batch_create", "tsmi_batch_destroy", "tsmi_batch_set_state",
     "tsmi_batch_set_exp_potential", "tsmi_batch_evolve", "tsmi_batch_get_state", "tsmi_batch_tune",
     "tsmi_batch_info", "tsmi_batch_synchronize", "tsmi_kernel_timing", "tsmi_kernel_time", "tsmi_squared_norm", "tsmi_energy", "tsmi_expected_values",
-    "tsmi_get_sample", "tsmi_get_state_device", "tsmi_synchronize", "tsmi_geometry", "tsmi_last_error",
+    "tsmi_get_sample", "tsmi_get_state_device", "tsmi_synchronize", "tsmi_geometry", "tsmi_plan_geometry",
+    "tsmi_last_error",
     "tsmi_pair_create", "tsmi_pair_destroy", "tsmi_pair_set_state", "tsmi_pair_set_exp_potential",
     "tsmi_pair_set_potential", "tsmi_pair_evolve", "tsmi_pair_step", "tsmi_pair_rabi", "tsmi_pair_get_state",
     "tsmi_pair_energy",
@@ -88,6 +89,7 @@ def load():
     L.tsmi_get_state_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.tsmi_evolve.argtypes = [C.c_void_p, C.c_int32]
     L.tsmi_step.argtypes = [C.c_void_p, C.c_int32]
+    L.tsmi_run_steps.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     L.tsmi_evolve_checkpointed.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     L.tsmi_adjoint.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
     L.tsmi_adjoint_device.argtypes = [C.c_void_p] + [C.c_void_p] * 5
@@ -109,6 +111,7 @@ def load():
     L.tsmi_get_sample.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _dp, _dp]
     L.tsmi_synchronize.argtypes = [C.c_void_p]
     L.tsmi_geometry.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    L.tsmi_plan_geometry.argtypes = [C.POINTER(Desc), C.POINTER(C.c_int32)]
     L.tsmi_pair_create.argtypes = [C.POINTER(PairDesc), C.POINTER(C.c_void_p)]
     L.tsmi_pair_destroy.argtypes = [C.c_void_p]
     L.tsmi_pair_destroy.restype = None
