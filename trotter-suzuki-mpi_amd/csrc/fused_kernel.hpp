@@ -5,10 +5,10 @@
 // series: pipeline A runs the nine sub-passes of step n on the input rows, and
 // every row it emits (4 rows behind its input) is the input row of pipeline B,
 // which runs step n+1 and emits 8 rows behind the input.  Each state row is read
-// once and written once for two steps; the E row is read by both pipelines, 4
-// rows apart.  (Keeping pipeline A's E rows in an LDS ring for pipeline B cut
-// the fetched bytes by a fifth but made the pass slower at 8192^2: the second
-// read is mostly served by the caches.)
+// once and written once for two steps.  FP32 segments strictly inside the tile
+// (all but the first and last strip and segment) run step2_stream_f32, which
+// keeps every row in fixed registers and reads each E row once; the other
+// segments run step2_wave, whose pipelines read each E row twice, 4 rows apart.
 //
 // Two steps on a tile with closed edges equal two single steps exactly when no
 // halo exchange is due between them: on closed axes always; on periodic axes
@@ -18,6 +18,8 @@
 // only: imaginary time renormalises between steps.
 #pragma once
 #include "step_kernel.hpp"
+
+#include <utility>
 
 namespace tsmi {
 
@@ -78,6 +80,115 @@ __device__ __forceinline__ void pipe_beat_fast(Pipe<T, VEC> &q, Row<T, VEC> in, 
     vpair_fast<TP & 1, false, M>(q.p4, o3, kV);
     out = q.p4;
     q.p4 = o3;
+}
+
+template <typename F, int... K>
+__device__ __forceinline__ void unroll_beats(F &&f, std::integer_sequence<int, K...>) {
+    (f(std::integral_constant<int, K>()), ...);
+}
+
+// The FP32 stream of an interior segment of an interior strip (phase plane, one
+// scaled form on both axes): the segment [y_lo, y_hi) lies strictly inside the
+// tile, so its first and last 8 rows are apron whatever they hold.  Every beat
+// is the fast beat: rows before the segment start as zeros in the pipelines and
+// rows past it load as zeros (the buffer range check), which changes only apron
+// rows (the dependency cone of two steps is 8 rows).  Rows stay in place: with
+// state rows loaded D beats ahead of pipeline A, row r lives in slot r mod
+// (D + 8) from its load until it leaves pipeline B (8 beats after entering A),
+// and the phase row r in slot r mod (D + 8) from its load until pipeline B's use
+// 4 beats after pipeline A's, so pipeline B reads no phase row from memory.
+// The loop body is one period of D + 8 beats, so every row keeps its registers
+// across the back edge (no copies between pipeline stages).  Beat j (relative
+// to y_lo): pipeline A takes row j, uses phase row j-2 and completes row j-4,
+// which pipeline B takes (phase row j-6), completing row j-8; then rows j+D
+// (state) and j+D-2 (phase) are loaded and the row j-8 is stored if it is an
+// output row.
+#ifndef TSMI_STREAM_AHEAD
+#define TSMI_STREAM_AHEAD 4
+#endif
+#ifndef TSMI_STREAM_LOADS_FIRST
+#define TSMI_STREAM_LOADS_FIRST 0
+#endif
+#ifndef TSMI_STREAM
+#define TSMI_STREAM 1
+#endif
+constexpr int kStreamAhead = TSMI_STREAM_AHEAD;
+
+template <int TP, bool LHY, int MODE>
+__device__ __forceinline__ void stream_pipe(Row<float, 2> &r0, Row<float, 2> &r1, Row<float, 2> &r2,
+                                            Row<float, 2> &r3, Row<float, 2> &r4, const Row<float, 2> &e, float kH,
+                                            float kV, float c, float cl, double mag) {
+    constexpr int M = MODE;
+    vpair_fast<(TP + 1) & 1, false, M>(r1, r0, kV);
+    hpass_fast<(TP + 1) & 1, false, M>(r1, kH);
+    vpair_fast<(TP + 1) & 1, false, M>(r2, r1, kV);
+    hpass_fast<(TP + 1) & 1, false, M>(r2, kH);
+    ppass<false, LHY, true>(r2, e, c, cl, mag);
+    hpass_fast<(TP + 1) & 1, false, M>(r2, kH);
+    vpair_fast<TP & 1, false, M>(r3, r2, kV);
+    hpass_fast<(TP + 1) & 1, false, M>(r3, kH);
+    vpair_fast<TP & 1, false, M>(r4, r3, kV);
+}
+
+template <bool LHY, int MODE>
+__device__ __forceinline__ void step2_stream_f32(__amdgpu_buffer_rsrc_t r_in_re, __amdgpu_buffer_rsrc_t r_in_im,
+                                                 __amdgpu_buffer_rsrc_t r_ph, __amdgpu_buffer_rsrc_t r_out_re,
+                                                 __amdgpu_buffer_rsrc_t r_out_im, unsigned v_ld, unsigned v_st,
+                                                 unsigned row_bytes, int rows, int out_lo, int out_hi, float kH,
+                                                 float kV, float c, float cl, double mag) {
+    constexpr int D = kStreamAhead;
+    constexpr int S = D + 8;
+    static_assert(D >= 2 && S % 2 == 0, "pair parities need an even period");
+    Row<float, 2> s[S];
+    Row<float, 2> e[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        s[i].re[0] = s[i].re[1] = s[i].im[0] = s[i].im[1] = 0.f;
+        e[i].re[0] = e[i].re[1] = e[i].im[0] = e[i].im[1] = 0.f;
+    }
+    auto row_off = [&](int r) { return (unsigned)r * row_bytes; };
+    auto load_state = [&](int r, Row<float, 2> &d) {
+        const unsigned off = v_ld + row_off(r);
+        buf_load<float, 2>(r_in_re, off, 0u, d.re);
+        buf_load<float, 2>(r_in_im, off, 0u, d.im);
+    };
+    auto load_phase = [&](int r, Row<float, 2> &d) { buf_load<float, 2>(r_ph, v_ld + row_off(r), 0u, d.re); };
+#pragma unroll
+    for (int r = 0; r < D; ++r) load_state(r, s[r]);
+#pragma unroll
+    for (int r = 0; r < D - 2; ++r) load_phase(r, e[r]);
+    const int beats = (rows + S - 1) / S * S;
+    for (int j0 = 0; j0 < beats; j0 += S) {
+        unroll_beats(
+            [&](auto kk) {
+                constexpr int k = decltype(kk)::value;
+                constexpr int TP = k & 1;
+                const int j = j0 + k;
+                stream_pipe<TP, LHY, MODE>(s[k % S], s[(k + S - 1) % S], s[(k + S - 2) % S], s[(k + S - 3) % S],
+                                           s[(k + S - 4) % S], e[(k + S - 2) % S], kH, kV, c, cl, mag);
+                stream_pipe<TP, LHY, MODE>(s[(k + S - 4) % S], s[(k + S - 5) % S], s[(k + S - 6) % S],
+                                           s[(k + S - 7) % S], s[(k + S - 8) % S], e[(k + S - 6) % S], kH, kV, c,
+                                           cl, mag);
+                // row j-8 leaves pipeline B; its slot takes row j+D
+                Row<float, 2> &out = s[(k + S - 8) % S];
+                const int y = j - 8;
+                const unsigned off = (y >= out_lo && y < out_hi) ? v_st + row_off(y) : kOffNone;
+                if (TSMI_STREAM_LOADS_FIRST) {
+                    const float o0 = out.re[0], o1 = out.re[1], o2 = out.im[0], o3 = out.im[1];
+                    load_phase(j + D - 2, e[(k + D - 2) % S]);
+                    load_state(j + D, out);
+                    const float ore[2] = {o0, o1}, oim[2] = {o2, o3};
+                    buf_store<float, 2>(r_out_re, off, 0u, ore);
+                    buf_store<float, 2>(r_out_im, off, 0u, oim);
+                } else {
+                    buf_store<float, 2>(r_out_re, off, 0u, out.re);
+                    buf_store<float, 2>(r_out_im, off, 0u, out.im);
+                    load_state(j + D, out);
+                    load_phase(j + D - 2, e[(k + D - 2) % S]);
+                }
+            },
+            std::make_integer_sequence<int, S>());
+    }
 }
 
 // Two real-time steps of one wavefront's strip segment.  p.seg_rows, p.n_strips
@@ -253,11 +364,40 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
 // FP32 with E as one phase plane and one scaled form on both axes: at most 128
 // VGPRs (4 waves per SIMD; the compiler fits them without spills).  The host
 // sizes the segments for the occupancy of the instance it launches.
+#ifndef TSMI_STEP2_OCC
+#define TSMI_STEP2_OCC 4
+#endif
 template <typename T, int VEC, bool LHY, int MODE, bool PH = false>
-__global__ void __launch_bounds__(256, (sizeof(T) == 4 && PH && MODE >= 0) ? 4 : 1) step2_kernel(StepParams<T> p) {
+__global__ void __launch_bounds__(256, (sizeof(T) == 4 && PH && MODE >= 0) ? TSMI_STEP2_OCC : 1) step2_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
+    if constexpr (TSMI_STREAM && sizeof(T) == 4 && VEC == 2 && PH && MODE >= 0) {
+        constexpr int AP = 2 * kApron;
+        constexpr int SW = kWave * VEC - 2 * AP;
+        const int strip = wave % p.n_strips;
+        const int seg = wave / p.n_strips;
+        const int X0 = strip * SW - AP;
+        const int y_out_lo = seg * p.seg_rows;
+        const int y_out_hi = min(p.H, y_out_lo + p.seg_rows);
+        const int y_lo = y_out_lo - AP;
+        const int y_hi = y_out_hi + AP;
+        if (X0 >= 0 && X0 + kWave * VEC <= p.W && y_lo > 0 && y_hi < p.H) {
+            const int xl = X0 + lane * VEC;
+            const bool own = lane >= AP / VEC && lane < kWave - AP / VEC;
+            const long seg0 = (long)y_lo * p.pitch;
+            const unsigned seg_bytes = (unsigned)((long)(y_hi - y_lo) * p.pitch * (long)sizeof(T));
+            const unsigned v_ld = (unsigned)xl * (unsigned)sizeof(T);
+            const unsigned v_st = own ? v_ld : kOffNone;
+            const T kH = MODE == 0 ? p.tH : p.mH, kV = MODE == 0 ? p.tV : p.mV;
+            step2_stream_f32<LHY, MODE>(make_rsrc(p.in_re + seg0, seg_bytes), make_rsrc(p.in_im + seg0, seg_bytes),
+                                        make_rsrc(p.ph + seg0, seg_bytes), make_rsrc(p.out_re + seg0, seg_bytes),
+                                        make_rsrc(p.out_im + seg0, seg_bytes), v_ld, v_st,
+                                        (unsigned)p.pitch * (unsigned)sizeof(T), y_hi - y_lo, AP,
+                                        AP + (y_out_hi - y_out_lo), kH, kV, p.c_nl, p.c_lhy, p.mag);
+            return;
+        }
+    }
     step2_wave<T, VEC, LHY, MODE, PH>(p, wave, lane);
 }
 

@@ -368,24 +368,25 @@ __device__ __forceinline__ void sincos_row(const T (&x)[VEC], T (&s)[VEC], T (&c
     T m = fabs(x[0]);
 #pragma unroll
     for (int k = 1; k < VEC; ++k) m = fmax(m, fabs(x[k]));
-    if (wave_all_le(m, T(0.015625))) {
+    // the short polynomials always, the general routine over them in the rare
+    // wave with a large angle (no copies on the common path)
 #pragma unroll
-        for (int k = 0; k < VEC; ++k) {
-            const T r = x[k], r2 = r * r;
-            if constexpr (sizeof(T) == 4) {
-                s[k] = fmaf(r * r2, -0.166666667f, r);
-                c[k] = fmaf(r2, fmaf(r2, 0.0416666667f, -0.5f), 1.0f);
-            } else {
-                T ps = fma(r2, -1.0 / 5040.0, 1.0 / 120.0);
-                ps = fma(r2, ps, -1.0 / 6.0);
-                s[k] = fma(r * r2, ps, r);
-                T pc = fma(r2, 1.0 / 40320.0, -1.0 / 720.0);
-                pc = fma(r2, pc, 1.0 / 24.0);
-                pc = fma(r2, pc, -0.5);
-                c[k] = fma(r2, pc, 1.0);
-            }
+    for (int k = 0; k < VEC; ++k) {
+        const T r = x[k], r2 = r * r;
+        if constexpr (sizeof(T) == 4) {
+            s[k] = fmaf(r * r2, -0.166666667f, r);
+            c[k] = fmaf(r2, fmaf(r2, 0.0416666667f, -0.5f), 1.0f);
+        } else {
+            T ps = fma(r2, -1.0 / 5040.0, 1.0 / 120.0);
+            ps = fma(r2, ps, -1.0 / 6.0);
+            s[k] = fma(r * r2, ps, r);
+            T pc = fma(r2, 1.0 / 40320.0, -1.0 / 720.0);
+            pc = fma(r2, pc, 1.0 / 24.0);
+            pc = fma(r2, pc, -0.5);
+            c[k] = fma(r2, pc, 1.0);
         }
-    } else {
+    }
+    if (__builtin_expect(!wave_all_le(m, T(0.015625)), 0)) {
 #pragma unroll
         for (int k = 0; k < VEC; ++k) sincos_t(x[k], &s[k], &c[k]);
     }
@@ -396,44 +397,51 @@ __device__ __forceinline__ void expneg_row(const T (&x)[VEC], T (&e)[VEC]) {
     T m = fabs(x[0]);
 #pragma unroll
     for (int k = 1; k < VEC; ++k) m = fmax(m, fabs(x[k]));
-    if (wave_all_le(m, T(0.015625))) {
 #pragma unroll
-        for (int k = 0; k < VEC; ++k) {
-            const T r = -x[k];
-            if constexpr (sizeof(T) == 4) {
-                float q = fmaf(r, 0.0416666667f, 0.166666667f);
-                q = fmaf(r, q, 0.5f);
-                q = fmaf(r, q, 1.0f);
-                e[k] = fmaf(r, q, 1.0f);
-            } else {
-                double q = fma(r, 1.0 / 40320.0, 1.0 / 5040.0);
-                q = fma(r, q, 1.0 / 720.0);
-                q = fma(r, q, 1.0 / 120.0);
-                q = fma(r, q, 1.0 / 24.0);
-                q = fma(r, q, 1.0 / 6.0);
-                q = fma(r, q, 0.5);
-                q = fma(r, q, 1.0);
-                e[k] = fma(r, q, 1.0);
-            }
+    for (int k = 0; k < VEC; ++k) {
+        const T r = -x[k];
+        if constexpr (sizeof(T) == 4) {
+            float q = fmaf(r, 0.0416666667f, 0.166666667f);
+            q = fmaf(r, q, 0.5f);
+            q = fmaf(r, q, 1.0f);
+            e[k] = fmaf(r, q, 1.0f);
+        } else {
+            double q = fma(r, 1.0 / 40320.0, 1.0 / 5040.0);
+            q = fma(r, q, 1.0 / 720.0);
+            q = fma(r, q, 1.0 / 120.0);
+            q = fma(r, q, 1.0 / 24.0);
+            q = fma(r, q, 1.0 / 6.0);
+            q = fma(r, q, 0.5);
+            q = fma(r, q, 1.0);
+            e[k] = fma(r, q, 1.0);
         }
-    } else {
+    }
+    if (__builtin_expect(!wave_all_le(m, T(0.015625)), 0)) {
 #pragma unroll
         for (int k = 0; k < VEC; ++k) e[k] = exp_t(-x[k]);
     }
 }
 
-// mag (cos, sin) of the phase form, rounded to the kernel precision.  In FP32,
-// |corr| differs from 1 by less than half an ulp near 1, so cos * |corr| rounded
-// alone would drop it at almost every site (a norm bias of ~6e-9 per step);
-// renormalising (cos, sin) in FP64 first (first order in cos^2 + sin^2 - 1)
-// dithers that rounding and leaves no bias above ~1e-10 per step.
+// mag (cos, sin) of the phase form, rounded to the kernel precision.  In FP32
+// the factor is k = mag (1 + e/2), e = 1 - cos^2 - sin^2 (first-order
+// renormalisation of the rounded pair), and each output is rounded once from
+// cos * k: when mag is within an ulp of 1, cos * mag rounded alone would drop
+// it at almost every site (a uniform norm bias of ~6e-9 per step), while the
+// per-site e dithers the rounding.  mag = m1 + m2 in two floats, so no
+// rounding of mag is applied uniformly either: cos * k = fma(cos, m1, cos * u)
+// with u = m2 + m1 e / 2, a term ~1e-7 of the result whose own FP32 rounding
+// stays below 1e-14.  1 - cos^2 - sin^2 comes from two fused multiply-adds
+// (each square exact inside its fma).
 template <typename T>
 __device__ __forceinline__ void phase_factor(T cs, T sn, double mag, T &c2, T &s2) {
     if constexpr (sizeof(T) == 4) {
-        const double cd = cs, sd = sn;
-        const double kk = mag * (1.5 - 0.5 * (cd * cd + sd * sd));
-        c2 = T(cd * kk);
-        s2 = T(sd * kk);
+        const float m1 = (float)mag;
+        const float m2 = (float)(mag - (double)m1);
+        float e = fmaf(-cs, cs, 1.0f);
+        e = fmaf(-sn, sn, e);
+        const float u = fmaf(0.5f * m1, e, m2);
+        c2 = fmaf(cs, m1, cs * u);
+        s2 = fmaf(sn, m1, sn * u);
     } else {
         c2 = cs * mag;
         s2 = sn * mag;

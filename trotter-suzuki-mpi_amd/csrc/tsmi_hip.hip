@@ -15,6 +15,7 @@
 #include <complex>
 #include <cstring>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "adjoint_kernel.hpp"
@@ -1121,23 +1122,33 @@ int tsmi_get_sample(tsmi_ctx *c, size_t stride, size_t x, size_t y, size_t w, si
     int st = current_div(c, &div);
     if (st) return st;
     const size_t es = (size_t)c->esz;
-    std::vector<unsigned char> tr(w * h * es), ti(w * h * es);
     const size_t off = (y * (size_t)c->pitch + x) * es;
-    TSMI_HIP(hipMemcpy2DAsync(tr.data(), w * es, (const char *)c->psi_re[c->cur] + off, (size_t)c->pitch * es, w * es, h,
-                              hipMemcpyDeviceToHost, c->stream));
-    TSMI_HIP(hipMemcpy2DAsync(ti.data(), w * es, (const char *)c->psi_im[c->cur] + off, (size_t)c->pitch * es, w * es, h,
-                              hipMemcpyDeviceToHost, c->stream));
-    TSMI_HIP(hipStreamSynchronize(c->stream));
     const bool scale = c->d.imag_time && c->div_pending;
+    if (es == 8 && !scale) {
+        // FP64 with nothing to apply: straight into the caller's rows
+        TSMI_HIP(hipMemcpy2DAsync(re, stride * 8, (const char *)c->psi_re[c->cur] + off, (size_t)c->pitch * 8, w * 8,
+                                  h, hipMemcpyDeviceToHost, c->stream));
+        TSMI_HIP(hipMemcpy2DAsync(im, stride * 8, (const char *)c->psi_im[c->cur] + off, (size_t)c->pitch * 8, w * 8,
+                                  h, hipMemcpyDeviceToHost, c->stream));
+        TSMI_HIP(hipStreamSynchronize(c->stream));
+        return TSMI_OK;
+    }
+    // staging (left uninitialised: the copies fill it)
+    std::unique_ptr<unsigned char[]> tr(new unsigned char[w * h * es]), ti(new unsigned char[w * h * es]);
+    TSMI_HIP(hipMemcpy2DAsync(tr.get(), w * es, (const char *)c->psi_re[c->cur] + off, (size_t)c->pitch * es, w * es,
+                              h, hipMemcpyDeviceToHost, c->stream));
+    TSMI_HIP(hipMemcpy2DAsync(ti.get(), w * es, (const char *)c->psi_im[c->cur] + off, (size_t)c->pitch * es, w * es,
+                              h, hipMemcpyDeviceToHost, c->stream));
+    TSMI_HIP(hipStreamSynchronize(c->stream));
     for (size_t j = 0; j < h; ++j)
         for (size_t i = 0; i < w; ++i) {
             double vr, vi;
             if (es == 8) {
-                vr = ((const double *)tr.data())[j * w + i];
-                vi = ((const double *)ti.data())[j * w + i];
+                vr = ((const double *)tr.get())[j * w + i];
+                vi = ((const double *)ti.get())[j * w + i];
             } else {
-                vr = ((const float *)tr.data())[j * w + i];
-                vi = ((const float *)ti.data())[j * w + i];
+                vr = ((const float *)tr.get())[j * w + i];
+                vi = ((const float *)ti.get())[j * w + i];
             }
             if (scale) { vr /= div; vi /= div; }   // src/cpukernel.cpp:559-560
             re[j * stride + i] = vr;
