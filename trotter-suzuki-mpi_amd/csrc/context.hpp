@@ -48,7 +48,8 @@ struct tsmi_ctx {
     double *div = nullptr;
     double *partials = nullptr;
     int *part_n = nullptr;
-    double *obs_partials = nullptr;
+    double *obs_partials = nullptr;      // kObsBlocksMax * OBS_COUNT block partials
+    double *obs_out = nullptr;           // kObsOut results of the last observables pass
     int obs_blocks = 0;
     int cur = 0, div_slot = 0;
     bool div_pending = false, has_state = false, has_e = false, has_v = false;

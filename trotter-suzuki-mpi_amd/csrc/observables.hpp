@@ -1,12 +1,21 @@
 // observables.hpp -- FP64 reductions over the tile: norm, energies and the
-// State expected values, in one pass.
+// State expected values, in one pass over HBM.
 //
-// Each block sums a stripe of rows with per-thread Neumaier compensation, then
-// the block's threads are summed in thread order, and the host sums the block
-// partials in block order, so the result does not depend on scheduling.
 // The quantities are those of Solver::calculate_energy_expected_values
 // (src/solver.cpp:320-420, single component, angular velocity 0) and of
 // State::calculate_expected_values (src/model.cpp:408-455).
+//
+// Layout of the pass: a thread owns one column of the inner region and walks a
+// band of rows, keeping the five rows y-2..y+2 of its column in registers (the
+// y stencils), so every state value is loaded once per band; the x stencils
+// read the neighbouring columns, which the neighbouring lanes of the same load
+// bring into the cache.  Each thread sums 32 rows plainly and adds the chunk
+// sums with Neumaier compensation; a wave combines its lanes in a fixed xor
+// tree, a block its four waves in order, and obs_finalize_kernel sums the block
+// partials with Neumaier compensation in block order -- the result does not
+// depend on scheduling.  The sums are of the stored values; a pending
+// imaginary-time divisor is applied to the final quantities (every sum is
+// homogeneous in psi: degree 2, 4 for the interaction sum, 5 for LHY).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -27,20 +36,29 @@ enum ObsIndex {
     OBS_COUNT
 };
 
+// what a pass sums: the norm alone, the energy sums (N .. LHY), or everything
+enum ObsKind { OBS_KIND_NORM = 0, OBS_KIND_ENERGY = 1, OBS_KIND_ALL = 2 };
+
 constexpr int kObsThreads = 256;
+constexpr int kObsChunk = 32;            // rows summed plainly before a compensated add
+// Block partials a pass may write: the grid is (column blocks) x (row bands)
+// with at most this many blocks, so the inner region may be at most
+// kObsBlocksMax * kObsThreads columns wide.
+constexpr int kObsBlocksMax = 4096;
+constexpr int kObsOut = 16;              // obs_finalize_kernel's outputs
 
 template <typename T>
 struct ObsParams {
     const T *re, *im;
     const double *v;       // raw potential, same pitch (may be null: potential sums are 0)
-    const double *xc, *yc; // mapped coordinates of tile columns / rows
+    const double *xc, *yc; // mapped coordinates of tile columns / rows (OBS_KIND_ALL only)
     int W, H, pitch;
     int x0, x1, y0, y1;    // inner region
     int kx0, kx1, ky0, ky1;// stencil margin
     int rows_free;         // 1: the margin ignores the row bounds (energy with H == 1)
-    double div;            // loaded values are divided by this (pending renormalisation)
+    int band;              // rows per block row
     double dx, dy;
-    double *partials;      // [gridDim.x][OBS_COUNT]
+    double *partials;      // [gridDim.y][gridDim.x][OBS_COUNT]
 };
 
 __device__ __forceinline__ void nsum(double &s, double &c, double v) {
@@ -49,69 +67,192 @@ __device__ __forceinline__ void nsum(double &s, double &c, double v) {
     s = t;
 }
 
-template <typename T>
+template <int KIND>
+constexpr int obs_used() {
+    return KIND == OBS_KIND_NORM ? 1 : KIND == OBS_KIND_ENERGY ? OBS_LHY + 1 : OBS_COUNT;
+}
+
+template <typename T, int KIND, bool LHY>
 __global__ void __launch_bounds__(kObsThreads) observables_kernel(ObsParams<T> p) {
-    double s[OBS_COUNT], c[OBS_COUNT];
+    constexpr int NQ = obs_used<KIND>();
+    double acc[NQ], s[NQ], c[NQ];
 #pragma unroll
-    for (int i = 0; i < OBS_COUNT; ++i) { s[i] = 0.0; c[i] = 0.0; }
+    for (int i = 0; i < NQ; ++i) acc[i] = s[i] = c[i] = 0.0;
+    const int x = p.x0 + (int)blockIdx.x * kObsThreads + (int)threadIdx.x;
+    const bool col = x < p.x1;
+    const int xs = col ? x : p.x0;                 // lanes past the region load a valid column
+    const int yb0 = p.y0 + (int)blockIdx.y * p.band;
+    const int yb1 = min(p.y1, yb0 + p.band);
+    const long P = p.pitch;
+    const bool col_margin = col && x >= p.kx0 && x < p.kx1;
     const double c1 = -1. / 12., c2 = 4. / 3., c3 = -2.5;
     const double d1 = 1. / 6., d2 = -1., d3 = 0.5, d4 = 1. / 3.;
-    const bool has_v = p.v != nullptr;
-    for (int y = p.y0 + (int)blockIdx.x; y < p.y1; y += (int)gridDim.x) {
-        const long row = (long)y * p.pitch;
-        const double Y = p.yc[y];
-        const bool row_margin = p.rows_free || (y >= p.ky0 && y < p.ky1);
-        for (int x = p.x0 + (int)threadIdx.x; x < p.x1; x += kObsThreads) {
-            auto R = [&](long o) { return double(p.re[o]) / p.div; };
-            auto I = [&](long o) { return double(p.im[o]) / p.div; };
-            const long k = row + x;
-            const double pr = R(k), pi = I(k);
-            const double n2 = pr * pr + pi * pi;
-            const double X = p.xc[x];
-            nsum(s[OBS_N], c[OBS_N], n2);
-            if (has_v) nsum(s[OBS_POT], c[OBS_POT], n2 * p.v[k]);
-            nsum(s[OBS_INT], c[OBS_INT], n2 * n2);
-            nsum(s[OBS_LHY], c[OBS_LHY], pow(n2, 2.5));
-            nsum(s[OBS_X], c[OBS_X], n2 * X);
-            nsum(s[OBS_XX], c[OBS_XX], n2 * X * X);
-            nsum(s[OBS_Y], c[OBS_Y], n2 * Y);
-            nsum(s[OBS_YY], c[OBS_YY], n2 * Y * Y);
-            if (!(row_margin && x >= p.kx0 && x < p.kx1)) continue;
-            nsum(s[OBS_NK], c[OBS_NK], n2);
-            const long P = p.pitch;
-            const double rr = R(k + 2), ir = I(k + 2), r1 = R(k + 1), i1 = I(k + 1);
-            const double l1r = R(k - 1), l1i = I(k - 1), l2r = R(k - 2), l2i = I(k - 2);
-            const double lxr = c1 * rr + c2 * r1 + c2 * l1r + c1 * l2r + c3 * pr;
-            const double lxi = c1 * ir + c2 * i1 + c2 * l1i + c1 * l2i + c3 * pi;
-            nsum(s[OBS_LAPX], c[OBS_LAPX], pr * lxr + pi * lxi);
-            const double dxr = d4 * r1 + d3 * pr + d2 * l1r + d1 * l2r;
-            const double dxi = d4 * i1 + d3 * pi + d2 * l1i + d1 * l2i;
-            nsum(s[OBS_PX], c[OBS_PX], pr * dxi - pi * dxr);
-            if (p.H > 1) {
-                const double dr = R(k + P), di = I(k + P), ddr = R(k + 2 * P), ddi = I(k + 2 * P);
-                const double ur = R(k - P), ui = I(k - P), uur = R(k - 2 * P), uui = I(k - 2 * P);
-                const double lyr = c1 * ddr + c2 * dr + c2 * ur + c1 * uur + c3 * pr;
-                const double lyi = c1 * ddi + c2 * di + c2 * ui + c1 * uui + c3 * pi;
-                nsum(s[OBS_LAPY], c[OBS_LAPY], pr * lyr + pi * lyi);
-                // "up" is row y-1 (src/model.cpp:448)
-                const double dyr = d4 * ur + d3 * pr + d2 * dr + d1 * ddr;
-                const double dyi = d4 * ui + d3 * pi + d2 * di + d1 * ddi;
-                nsum(s[OBS_PY], c[OBS_PY], pr * dyi - pi * dyr);
-                const double zr = Y / p.dx * dxr + X / p.dy * dyr;
-                const double zi = Y / p.dx * dxi + X / p.dy * dyi;
-                nsum(s[OBS_LZ], c[OBS_LZ], pr * zi - pi * zr);
+    const double X = (KIND == OBS_KIND_ALL) ? p.xc[xs] : 0.0;
+    // rows outside the tile read as row 0 or H-1: such values only meet cells
+    // outside the stencil margin, whose sums do not use them
+    auto row_of = [&](int y) { return (long)min(max(y, 0), p.H - 1) * P; };
+    double wr[5], wi[5];                           // rows y-2 .. y+2 of this column
+    if (KIND != OBS_KIND_NORM) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long o = row_of(yb0 - 2 + i) + xs;
+            wr[i] = (double)p.re[o];
+            wi[i] = (double)p.im[o];
+        }
+    }
+    int chunk = 0;
+    for (int y = yb0; y < yb1; ++y) {
+        const long k = (long)y * P + xs;
+        double pr, pi;
+        if (KIND == OBS_KIND_NORM) {
+            pr = (double)p.re[k];
+            pi = (double)p.im[k];
+        } else {
+            const long o = row_of(y + 2) + xs;
+            wr[4] = (double)p.re[o];
+            wi[4] = (double)p.im[o];
+            pr = wr[2];
+            pi = wi[2];
+        }
+        const double n2 = col ? pr * pr + pi * pi : 0.0;
+        acc[OBS_N] += n2;
+        if constexpr (KIND != OBS_KIND_NORM) {
+            if (p.v) acc[OBS_POT] += n2 * p.v[k];
+            acc[OBS_INT] += n2 * n2;
+            if (LHY) acc[OBS_LHY] += n2 * n2 * sqrt(n2);
+            if constexpr (KIND == OBS_KIND_ALL) {
+                const double Y = p.yc[y];
+                acc[OBS_X] += n2 * X;
+                acc[OBS_XX] += n2 * X * X;
+                acc[OBS_Y] += n2 * Y;
+                acc[OBS_YY] += n2 * Y * Y;
+            }
+            const bool row_margin = p.rows_free || (y >= p.ky0 && y < p.ky1);
+            if (col_margin && row_margin) {
+                acc[OBS_NK] += n2;
+                const double rr = (double)p.re[k + 2], ir = (double)p.im[k + 2];
+                const double r1 = (double)p.re[k + 1], i1 = (double)p.im[k + 1];
+                const double l1r = (double)p.re[k - 1], l1i = (double)p.im[k - 1];
+                const double l2r = (double)p.re[k - 2], l2i = (double)p.im[k - 2];
+                const double lxr = c1 * rr + c2 * r1 + c2 * l1r + c1 * l2r + c3 * pr;
+                const double lxi = c1 * ir + c2 * i1 + c2 * l1i + c1 * l2i + c3 * pi;
+                acc[OBS_LAPX] += pr * lxr + pi * lxi;
+                double dxr = 0.0, dxi = 0.0;
+                if constexpr (KIND == OBS_KIND_ALL) {
+                    dxr = d4 * r1 + d3 * pr + d2 * l1r + d1 * l2r;
+                    dxi = d4 * i1 + d3 * pi + d2 * l1i + d1 * l2i;
+                    acc[OBS_PX] += pr * dxi - pi * dxr;
+                }
+                if (p.H > 1) {
+                    const double dr = wr[3], di = wi[3], ddr = wr[4], ddi = wi[4];
+                    const double ur = wr[1], ui = wi[1], uur = wr[0], uui = wi[0];
+                    const double lyr = c1 * ddr + c2 * dr + c2 * ur + c1 * uur + c3 * pr;
+                    const double lyi = c1 * ddi + c2 * di + c2 * ui + c1 * uui + c3 * pi;
+                    acc[OBS_LAPY] += pr * lyr + pi * lyi;
+                    if constexpr (KIND == OBS_KIND_ALL) {
+                        // "up" is row y-1 (src/model.cpp:448)
+                        const double Y = p.yc[y];
+                        const double dyr = d4 * ur + d3 * pr + d2 * dr + d1 * ddr;
+                        const double dyi = d4 * ui + d3 * pi + d2 * di + d1 * ddi;
+                        acc[OBS_PY] += pr * dyi - pi * dyr;
+                        const double zr = Y / p.dx * dxr + X / p.dy * dyr;
+                        const double zi = Y / p.dx * dxi + X / p.dy * dyi;
+                        acc[OBS_LZ] += pr * zi - pi * zr;
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                wr[i] = wr[i + 1];
+                wi[i] = wi[i + 1];
+            }
+        }
+        if (++chunk == kObsChunk) {
+            chunk = 0;
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                nsum(s[i], c[i], acc[i]);
+                acc[i] = 0.0;
             }
         }
     }
-    __shared__ double red[OBS_COUNT][kObsThreads];
+    // the thread's total, the wave's in a fixed xor tree, the block's in wave order
+    __shared__ double red[kObsThreads / 64][NQ];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < OBS_COUNT; ++i) red[i][threadIdx.x] = s[i] + c[i];
-    __syncthreads();
-    if (threadIdx.x < OBS_COUNT) {
-        double a = 0.0, cc = 0.0;
-        for (int t = 0; t < kObsThreads; ++t) nsum(a, cc, red[threadIdx.x][t]);
-        p.partials[(long)blockIdx.x * OBS_COUNT + threadIdx.x] = a + cc;
+    for (int i = 0; i < NQ; ++i) {
+        nsum(s[i], c[i], acc[i]);
+        double t = s[i] + c[i];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (lane == 0) red[wave][i] = t;
     }
+    __syncthreads();
+    if ((int)threadIdx.x < NQ) {
+        double a = 0.0, cc = 0.0;
+        for (int w = 0; w < kObsThreads / 64; ++w) nsum(a, cc, red[w][threadIdx.x]);
+        const long b = (long)blockIdx.y * gridDim.x + blockIdx.x;
+        p.partials[b * OBS_COUNT + threadIdx.x] = a + cc;
+    }
+}
+
+// What obs_finalize_kernel turns the sums into (src/solver.cpp:538-561 for the
+// energy, src/model.cpp:396-455 for the expected values).
+struct ObsFinal {
+    int kind;              // ObsKind
+    int nq;                // quantities summed
+    int blocks;            // block partials to add
+    int rows2d;            // H > 1
+    const double *div;     // pending imaginary-time divisor of the stored state (device; null: 1)
+    double mass, g, g_lhy, dx, dy, length_x;
+    int dim_x;
+};
+
+// One block of 64 * OBS_COUNT threads: wave i sums quantity i over the block
+// partials (lane-strided Neumaier in block order, then a fixed xor tree), and
+// thread 0 forms the outputs: OBS_KIND_NORM: out[0] = squared norm;
+// OBS_KIND_ENERGY: out[0..5] as tsmi_energy; OBS_KIND_ALL: out[0..5] as
+// tsmi_energy, out[6..15] as tsmi_expected_values.
+__global__ void __launch_bounds__(64 * OBS_COUNT) obs_finalize_kernel(const double *partials, ObsFinal f,
+                                                                      double *out) {
+    __shared__ double q[OBS_COUNT];
+    const int lane = threadIdx.x & 63, i = threadIdx.x >> 6;
+    if (i < f.nq) {
+        double a = 0.0, cc = 0.0;
+        for (int b = lane; b < f.blocks; b += 64) nsum(a, cc, partials[(long)b * OBS_COUNT + i]);
+        double t = a + cc;
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (lane == 0) q[i] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double dv = f.div ? *f.div : 1.0;
+    const double d2 = 1.0 / (dv * dv);
+    const double n = q[OBS_N] * d2;
+    if (f.kind == OBS_KIND_NORM) {
+        out[0] = n * f.dx * f.dy;
+        return;
+    }
+    // kinetic and potential terms are ratios of degree-2 sums: the divisor cancels
+    const double cost = -1. / (2. * f.mass);
+    const double kin = cost * (q[OBS_LAPX] / (f.dx * f.dx) + (f.rows2d ? q[OBS_LAPY] / (f.dy * f.dy) : 0.0)) /
+                       q[OBS_NK];
+    const double pot = q[OBS_POT] / q[OBS_N];
+    const double itr = 0.5 * f.g * (q[OBS_INT] * d2) / q[OBS_N];
+    const double lhy = 0.4 * f.g_lhy * (q[OBS_LHY] * d2 / dv) / q[OBS_N];
+    out[0] = kin + pot + itr + lhy;                                   // src/solver.cpp:538
+    out[1] = kin; out[2] = pot; out[3] = itr; out[4] = lhy;
+    out[5] = n * f.dy * f.length_x / (double)f.dim_x;                 // src/solver.cpp:561
+    if (f.kind != OBS_KIND_ALL) return;
+    const double qn = q[OBS_N];
+    const double ppx = -1. / f.dx, ppy = 1. / f.dy;                   // src/model.cpp:398-399
+    out[6] = n * (f.dx * f.dy);
+    out[7] = q[OBS_X] / qn; out[8] = q[OBS_XX] / qn;
+    out[9] = q[OBS_Y] / qn; out[10] = q[OBS_YY] / qn;
+    out[11] = -q[OBS_PX] * ppx / qn;
+    out[12] = -q[OBS_LAPX] * ppx * ppx / qn;
+    out[13] = -q[OBS_PY] * ppy / qn;
+    out[14] = -q[OBS_LAPY] * ppy * ppy / qn;
+    out[15] = q[OBS_LZ] / qn;
 }
 
 }  // namespace tsmi

@@ -14,8 +14,8 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
-#include <string>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "adjoint_kernel.hpp"
@@ -117,7 +117,6 @@ constexpr int kBlock = 256;          // 4 wavefronts per workgroup
 constexpr int kVec32 = TSMI_FP32_VEC;
 constexpr int kVec32I = TSMI_FP32_VEC_IMAG;   // the imaginary-time single step
 constexpr int kVec32F = TSMI_FP32_VEC2;
-constexpr int kObsBlocksMax = 1024;
 
 }  // namespace
 
@@ -143,7 +142,7 @@ int free_all(tsmi_ctx *c) {
         for (int k = 0; k < 2; ++k)
             if (c->psi_base[b][k]) (void)hipFree(c->psi_base[b][k]);
     void *ptrs[] = {c->e_base[0], c->e_base[1], c->e8[0], c->e8[1], c->ph_base, c->ph8, c->flag, c->v_base, c->xc,
-                    c->yc, c->div, c->partials, c->part_n, c->obs_partials, c->ck, c->adj, c->grad};
+                    c->yc, c->div, c->partials, c->part_n, c->obs_partials, c->obs_out, c->ck, c->adj, c->grad};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &k : c->ev)
@@ -665,49 +664,103 @@ int export_device(tsmi_ctx *c, const void *src, double *dst) {
     return TSMI_OK;
 }
 
+// Geometry of an observables pass over a tile of W x H cells (row pitch in
+// elements): the inner region and the 2-cell stencil margin next to closed
+// edges (src/solver.cpp:356-359, src/model.cpp:424-427).
+struct ObsGeom {
+    int W, H, pitch, x0, x1, y0, y1, kx0, kx1, ky0, ky1;
+};
+
+ObsGeom obs_geom(const tsmi_desc &d, int W, int H, int pitch) {
+    ObsGeom g;
+    g.W = W; g.H = H; g.pitch = pitch;
+    g.x0 = d.periodic_x ? d.halo_x : 0; g.x1 = W - g.x0;
+    g.y0 = d.periodic_y ? d.halo_y : 0; g.y1 = H - g.y0;
+    g.kx0 = g.x0 + (d.periodic_x ? 0 : 2); g.kx1 = g.x1 - (d.periodic_x ? 0 : 2);
+    g.ky0 = g.y0 + (d.periodic_y ? 0 : 2); g.ky1 = g.y1 - (d.periodic_y ? 0 : 2);
+    return g;
+}
+
+// Grid of a pass: one block per 256 columns of the inner region and row bands
+// of at least 16 rows, at most kObsBlocksMax blocks.  Returns false when the
+// inner region is wider than kObsBlocksMax * 256 columns.
+bool obs_grid(const ObsGeom &g, dim3 *grid, int *band) {
+    const int cb = (g.x1 - g.x0 + kObsThreads - 1) / kObsThreads;
+    if (cb < 1 || cb > kObsBlocksMax) return false;
+    const int rows = std::max(1, g.y1 - g.y0);
+    const int rb_max = kObsBlocksMax / cb;
+    *band = std::max(16, (rows + rb_max - 1) / rb_max);
+    *grid = dim3(cb, (rows + *band - 1) / *band);
+    return true;
+}
+
+// One pass of kind `kind` (ObsKind) over the planes, the block partials in
+// `partials` (kObsBlocksMax * OBS_COUNT doubles), then obs_finalize_kernel,
+// which writes kObsOut doubles to `out` (device).  `div` (device, may be null)
+// holds a pending imaginary-time divisor of the stored state.  No allocation,
+// no synchronisation.
 template <typename T>
-int run_observables(tsmi_ctx *c, bool energy_margin, double out[OBS_COUNT]) {
-    double div = 1.0;
-    int st = current_div(c, &div);
-    if (st) return st;
+int launch_observables(const tsmi_desc &d, const ObsGeom &g, const T *re, const T *im, const double *v,
+                       const double *xc, const double *yc, int kind, bool rows_free, double *partials,
+                       const double *div, double *out, hipStream_t stream, const char *who) {
+    dim3 grid;
+    int band = 0;
+    if (!obs_grid(g, &grid, &band))
+        return fail(TSMI_E_INVALID, std::string(who) + ": the inner region is wider than " +
+                                        std::to_string(kObsBlocksMax * kObsThreads) + " columns");
     ObsParams<T> p;
     std::memset(&p, 0, sizeof(p));
-    p.re = (const T *)c->psi_re[c->cur];
-    p.im = (const T *)c->psi_im[c->cur];
-    p.v = c->has_v ? c->v : nullptr;
-    p.xc = c->xc; p.yc = c->yc;
-    p.W = c->W; p.H = c->H; p.pitch = c->pitch;
-    p.x0 = c->ix0; p.x1 = c->ix1; p.y0 = c->iy0; p.y1 = c->iy1;
-    // 2-cell margin next to closed edges (src/solver.cpp:356-359, src/model.cpp:424-427)
-    p.kx0 = c->ix0 + (c->d.periodic_x ? 0 : 2);
-    p.kx1 = c->ix1 - (c->d.periodic_x ? 0 : 2);
-    p.ky0 = c->iy0 + (c->d.periodic_y ? 0 : 2);
-    p.ky1 = c->iy1 - (c->d.periodic_y ? 0 : 2);
-    p.rows_free = energy_margin && c->H == 1;
-    p.div = div;
-    p.dx = c->d.delta_x; p.dy = c->d.delta_y;
-    p.partials = c->obs_partials;
-    const int blocks = std::max(1, std::min(c->obs_blocks, c->iy1 - c->iy0));
-    hipLaunchKernelGGL((observables_kernel<T>), dim3(blocks), dim3(kObsThreads), 0, c->stream, p);
+    p.re = re; p.im = im; p.v = v; p.xc = xc; p.yc = yc;
+    p.W = g.W; p.H = g.H; p.pitch = g.pitch;
+    p.x0 = g.x0; p.x1 = g.x1; p.y0 = g.y0; p.y1 = g.y1;
+    p.kx0 = g.kx0; p.kx1 = g.kx1; p.ky0 = g.ky0; p.ky1 = g.ky1;
+    p.rows_free = rows_free ? 1 : 0;
+    p.band = band;
+    p.dx = d.delta_x; p.dy = d.delta_y;
+    p.partials = partials;
+    const bool lhy = kind != OBS_KIND_NORM && d.coupling_lhy != 0.0;
+    if (kind == OBS_KIND_NORM)
+        hipLaunchKernelGGL((observables_kernel<T, OBS_KIND_NORM, false>), grid, dim3(kObsThreads), 0, stream, p);
+    else if (kind == OBS_KIND_ENERGY && lhy)
+        hipLaunchKernelGGL((observables_kernel<T, OBS_KIND_ENERGY, true>), grid, dim3(kObsThreads), 0, stream, p);
+    else if (kind == OBS_KIND_ENERGY)
+        hipLaunchKernelGGL((observables_kernel<T, OBS_KIND_ENERGY, false>), grid, dim3(kObsThreads), 0, stream, p);
+    else if (lhy)
+        hipLaunchKernelGGL((observables_kernel<T, OBS_KIND_ALL, true>), grid, dim3(kObsThreads), 0, stream, p);
+    else
+        hipLaunchKernelGGL((observables_kernel<T, OBS_KIND_ALL, false>), grid, dim3(kObsThreads), 0, stream, p);
     TSMI_HIP(hipGetLastError());
-    std::vector<double> h((size_t)blocks * OBS_COUNT);
-    TSMI_HIP(hipMemcpyAsync(h.data(), c->obs_partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    TSMI_HIP(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < OBS_COUNT; ++i) {
-        double s = 0.0, comp = 0.0;
-        for (int b = 0; b < blocks; ++b) {
-            const double v = h[(size_t)b * OBS_COUNT + i];
-            const double t = s + v;
-            comp += (std::fabs(s) >= std::fabs(v)) ? (s - t) + v : (v - t) + s;
-            s = t;
-        }
-        out[i] = s + comp;
-    }
+    ObsFinal f;
+    std::memset(&f, 0, sizeof(f));
+    f.kind = kind;
+    f.nq = kind == OBS_KIND_NORM ? 1 : kind == OBS_KIND_ENERGY ? OBS_LHY + 1 : OBS_COUNT;
+    f.blocks = (int)(grid.x * grid.y);
+    f.rows2d = g.H > 1;
+    f.div = div;
+    f.mass = d.mass; f.g = d.coupling; f.g_lhy = d.coupling_lhy;
+    f.dx = d.delta_x; f.dy = d.delta_y; f.length_x = d.length_x; f.dim_x = d.dim_x;
+    hipLaunchKernelGGL(obs_finalize_kernel, dim3(1), dim3(64 * OBS_COUNT), 0, stream, (const double *)partials, f,
+                       out);
+    TSMI_HIP(hipGetLastError());
     return TSMI_OK;
 }
 
-int observables(tsmi_ctx *c, bool energy_margin, double out[OBS_COUNT]) {
-    return c->esz == 4 ? run_observables<float>(c, energy_margin, out) : run_observables<double>(c, energy_margin, out);
+// The pass over the context's current state; out: kObsOut doubles (host).
+int observables(tsmi_ctx *c, int kind, bool rows_free, double out[kObsOut]) {
+    const ObsGeom g = obs_geom(c->d, c->W, c->H, c->pitch);
+    const double *div = pending_div(c);
+    const double *v = c->has_v ? c->v : nullptr;
+    const int st = c->esz == 4
+        ? launch_observables<float>(c->d, g, (const float *)c->psi_re[c->cur], (const float *)c->psi_im[c->cur], v,
+                                    c->xc, c->yc, kind, rows_free, c->obs_partials, div, c->obs_out, c->stream,
+                                    "observables")
+        : launch_observables<double>(c->d, g, (const double *)c->psi_re[c->cur], (const double *)c->psi_im[c->cur],
+                                     v, c->xc, c->yc, kind, rows_free, c->obs_partials, div, c->obs_out, c->stream,
+                                     "observables");
+    if (st) return st;
+    TSMI_HIP(hipMemcpyAsync(out, c->obs_out, kObsOut * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TSMI_HIP(hipStreamSynchronize(c->stream));
+    return TSMI_OK;
 }
 
 }  // namespace
@@ -786,7 +839,8 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
         {(void **)&c->v_base, pd},
         {(void **)&c->xc, (size_t)c->W * sizeof(double)}, {(void **)&c->yc, (size_t)c->H * sizeof(double)},
         {(void **)&c->div, 2 * sizeof(double)}, {(void **)&c->partials, 2 * (size_t)c->n_waves * sizeof(double)},
-        {(void **)&c->part_n, 64}, {(void **)&c->obs_partials, (size_t)c->obs_blocks * OBS_COUNT * sizeof(double)},
+        {(void **)&c->part_n, 64}, {(void **)&c->obs_partials, (size_t)kObsBlocksMax * OBS_COUNT * sizeof(double)},
+        {(void **)&c->obs_out, kObsOut * sizeof(double)},
     };
     for (auto &a : allocs) {
         if (a.n == 0) continue;
@@ -1067,48 +1121,31 @@ int tsmi_kernel_time(tsmi_ctx *c, double out[4]) {
 int tsmi_squared_norm(tsmi_ctx *c, double *out) {
     if (!c || !out) return fail(TSMI_E_INVALID, "tsmi_squared_norm: null argument");
     if (!c->has_state) return fail(TSMI_E_INVALID, "tsmi_squared_norm: no state");
-    double o[OBS_COUNT];
-    const int st = observables(c, true, o);
+    double o[kObsOut];
+    const int st = observables(c, OBS_KIND_NORM, true, o);
     if (st) return st;
-    *out = o[OBS_N] * c->d.delta_x * c->d.delta_y;
+    *out = o[0];
     return TSMI_OK;
 }
 
 int tsmi_energy(tsmi_ctx *c, double out[6]) {
     if (!c || !out) return fail(TSMI_E_INVALID, "tsmi_energy: null argument");
     if (!c->has_state || !c->has_v) return fail(TSMI_E_INVALID, "tsmi_energy: state and potential must be set");
-    double o[OBS_COUNT];
-    const int st = observables(c, true, o);
+    double o[kObsOut];
+    // the margin of a one-row lattice ignores the row bounds (src/solver.cpp:356-359)
+    const int st = observables(c, OBS_KIND_ENERGY, c->H == 1, o);
     if (st) return st;
-    const tsmi_desc &d = c->d;
-    const double cost = -1. / (2. * d.mass);
-    const double kin = cost * (o[OBS_LAPX] / (d.delta_x * d.delta_x) +
-                               (c->H > 1 ? o[OBS_LAPY] / (d.delta_y * d.delta_y) : 0.0)) / o[OBS_NK];
-    const double pot = o[OBS_POT] / o[OBS_N];
-    const double itr = 0.5 * d.coupling * o[OBS_INT] / o[OBS_N];
-    const double lhy = 0.4 * d.coupling_lhy * o[OBS_LHY] / o[OBS_N];
-    out[0] = kin + pot + itr + lhy;                                // src/solver.cpp:538
-    out[1] = kin; out[2] = pot; out[3] = itr; out[4] = lhy;
-    out[5] = o[OBS_N] * d.delta_y * d.length_x / (double)d.dim_x;  // src/solver.cpp:561
+    std::memcpy(out, o, 6 * sizeof(double));
     return TSMI_OK;
 }
 
 int tsmi_expected_values(tsmi_ctx *c, double out[10]) {
     if (!c || !out) return fail(TSMI_E_INVALID, "tsmi_expected_values: null argument");
     if (!c->has_state) return fail(TSMI_E_INVALID, "tsmi_expected_values: no state");
-    double o[OBS_COUNT];
-    const int st = observables(c, false, o);
+    double o[kObsOut];
+    const int st = observables(c, OBS_KIND_ALL, false, o);
     if (st) return st;
-    const double n = o[OBS_N], dx = c->d.delta_x, dy = c->d.delta_y;
-    const double ppx = -1. / dx, ppy = 1. / dy;                   // src/model.cpp:398-399
-    out[0] = n * (dx * dy);
-    out[1] = o[OBS_X] / n; out[2] = o[OBS_XX] / n;
-    out[3] = o[OBS_Y] / n; out[4] = o[OBS_YY] / n;
-    out[5] = -o[OBS_PX] * ppx / n;
-    out[6] = -o[OBS_LAPX] * ppx * ppx / n;
-    out[7] = -o[OBS_PY] * ppy / n;
-    out[8] = -o[OBS_LAPY] * ppy * ppy / n;
-    out[9] = o[OBS_LZ] / n;
+    std::memcpy(out, o + 6, 10 * sizeof(double));
     return TSMI_OK;
 }
 
