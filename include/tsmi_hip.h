@@ -270,6 +270,25 @@ int tsmi_squared_norm(tsmi_ctx *ctx, double *out);
  * Needs tsmi_set_potential.  Syncs. */
 int tsmi_energy(tsmi_ctx *ctx, double out[6]);
 
+/* The energies of tsmi_energy for a state held in the caller's device memory,
+ * allocating nothing.  No counterpart in the reference: for callers that
+ * cannot allocate device memory -- code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc.  desc describes the lattice as for
+ * tsmi_create (precision 32: the planes hold floats, 64: doubles); psi_re,
+ * psi_im: the W x H tile (halo included on periodic axes) with a row pitch of
+ * `pitch` elements (pitch >= W); v: the raw potential as W x H doubles with the
+ * same pitch, or NULL (potential term 0); out: 6 doubles in device memory,
+ * written as tsmi_energy's out[0..5].  The block partials go to a fixed device
+ * array of the library (TSMI_NOALLOC_SCRATCH_BYTES on each GPU it runs on, held
+ * until the process ends, every element written before it is read); calls that
+ * overlap on one GPU race on it.  The work is queued on `stream` (a
+ * hipStream_t; NULL: the null stream) and not waited for.  Inner regions wider
+ * than TSMI_NOALLOC_MAX_COLUMNS columns are rejected (TSMI_E_INVALID). */
+#define TSMI_NOALLOC_SCRATCH_BYTES (4096 * 14 * 8)
+#define TSMI_NOALLOC_MAX_COLUMNS (4096 * 256)
+int tsmi_energy_device_noalloc(const tsmi_desc *desc, const void *psi_re, const void *psi_im, int64_t pitch,
+                               const double *v, double *out, void *stream);
+
 /* State::calculate_expected_values (src/model.cpp:385-536):
  * out[0..9] = norm2, <X>, <X^2>, <Y>, <Y^2>, <P_x>, <P_x^2>, <P_y>, <P_y^2>, <L_z>.
  * Syncs. */

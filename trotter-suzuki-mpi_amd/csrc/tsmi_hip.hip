@@ -15,6 +15,7 @@
 #include <complex>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -38,6 +39,14 @@ int fail(int code, const std::string &msg) {
 }
 
 }  // namespace
+
+// The device array behind tsmi_energy_device_noalloc: block partials of one
+// observables pass (kObsBlocksMax x OBS_COUNT doubles).  Each GPU holds its own
+// copy; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device.
+static __device__ __align__(256) double scratch[tsmi::kObsBlocksMax * tsmi::OBS_COUNT];
+static_assert(sizeof(scratch) == TSMI_NOALLOC_SCRATCH_BYTES, "the header states the scratch size");
+static_assert(TSMI_NOALLOC_MAX_COLUMNS == tsmi::kObsBlocksMax * tsmi::kObsThreads, "the header states the limit");
 
 namespace tsmi {
 
@@ -1137,6 +1146,41 @@ int tsmi_energy(tsmi_ctx *c, double out[6]) {
     if (st) return st;
     std::memcpy(out, o, 6 * sizeof(double));
     return TSMI_OK;
+}
+
+int tsmi_energy_device_noalloc(const tsmi_desc *desc, const void *psi_re, const void *psi_im, int64_t pitch,
+                               const double *v, double *out, void *stream) {
+    static const char *who = "tsmi_energy_device_noalloc";
+    if (!desc || !psi_re || !psi_im || !out) return fail(TSMI_E_INVALID, std::string(who) + ": null argument");
+    const tsmi_desc &d = *desc;
+    if (int st = check_desc(d, who)) return st;
+    const long long W = (long long)d.dim_x + (d.periodic_x ? 2 * d.halo_x : 0);
+    const long long H = (long long)d.dim_y + (d.periodic_y ? 2 * d.halo_y : 0);
+    if (pitch < W || pitch * H > (1LL << 40)) return fail(TSMI_E_INVALID, std::string(who) + ": bad row pitch");
+    // the scratch address on the current device, looked up once per device
+    static std::mutex mu;
+    static double *ptrs[64] = {};
+    int dev = 0;
+    TSMI_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(TSMI_E_INVALID, std::string(who) + ": device index above 63");
+    double *part = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (!ptrs[dev]) {
+            void *ptr = nullptr;
+            TSMI_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            ptrs[dev] = (double *)ptr;
+        }
+        part = ptrs[dev];
+    }
+    const ObsGeom g = obs_geom(d, (int)W, (int)H, (int)pitch);
+    hipStream_t s = (hipStream_t)stream;
+    // the margin of a one-row lattice ignores the row bounds (src/solver.cpp:356-359)
+    if (d.precision == 32)
+        return launch_observables<float>(d, g, (const float *)psi_re, (const float *)psi_im, v, nullptr, nullptr,
+                                         OBS_KIND_ENERGY, H == 1, part, nullptr, out, s, who);
+    return launch_observables<double>(d, g, (const double *)psi_re, (const double *)psi_im, v, nullptr, nullptr,
+                                      OBS_KIND_ENERGY, H == 1, part, nullptr, out, s, who);
 }
 
 int tsmi_expected_values(tsmi_ctx *c, double out[10]) {

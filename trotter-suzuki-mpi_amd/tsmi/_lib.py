@@ -22,7 +22,7 @@ EXPORTS = (
     "tsmi_set_potential_device", "tsmi_set_potential_exp_device", "tsmi_evolve", "tsmi_run_steps", "tsmi_step", "tsmi_evolve_checkpointed", "tsmi_adjoint",
     "tsmi_adjoint_device", "tsmi_batch_create", "tsmi_batch_destroy", "tsmi_batch_set_state",
     "tsmi_batch_set_exp_potential", "tsmi_batch_evolve", "tsmi_batch_get_state", "tsmi_batch_tune",
-    "tsmi_batch_info", "tsmi_batch_synchronize", "tsmi_kernel_timing", "tsmi_kernel_time", "tsmi_squared_norm", "tsmi_energy", "tsmi_expected_values",
+    "tsmi_batch_info", "tsmi_batch_synchronize", "tsmi_kernel_timing", "tsmi_kernel_time", "tsmi_squared_norm", "tsmi_energy", "tsmi_energy_device_noalloc", "tsmi_expected_values",
     "tsmi_get_sample", "tsmi_get_state_device", "tsmi_synchronize", "tsmi_geometry", "tsmi_plan_geometry",
     "tsmi_last_error",
     "tsmi_pair_create", "tsmi_pair_destroy", "tsmi_pair_set_state", "tsmi_pair_set_exp_potential",
@@ -107,6 +107,8 @@ def load():
     L.tsmi_kernel_time.argtypes = [C.c_void_p, _dp]
     L.tsmi_squared_norm.argtypes = [C.c_void_p, _dp]
     L.tsmi_energy.argtypes = [C.c_void_p, _dp]
+    L.tsmi_energy_device_noalloc.argtypes = [C.POINTER(Desc), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]
     L.tsmi_expected_values.argtypes = [C.c_void_p, _dp]
     L.tsmi_get_sample.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _dp, _dp]
     L.tsmi_synchronize.argtypes = [C.c_void_p]
