@@ -70,7 +70,7 @@ def parse():
     ap.add_argument("--no-api", action="store_true", help="skip timing Solver::evolve through the C++ API")
     ap.add_argument("--no-studies", action="store_true",
                     help="skip the precision study, the batch, adjoint and two-component measurements")
-    ap.add_argument("--only-study", choices=["precision_study", "batch", "adjoint", "pair"],
+    ap.add_argument("--only-study", choices=["precision_study", "batch", "adjoint", "pair", "observables"],
                     help="only run that study and print its JSON object (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (for counter calibration) and exit")
@@ -338,6 +338,34 @@ def adjoint_study():
     return out
 
 
+def observables_study(n: int = 8192, calls: int = 6):
+    """The read-outs (tsmi_energy, tsmi_expected_values, tsmi_squared_norm) on
+    the vortex lattice in both precisions: host-clock time per call (device
+    passes, the copy of the results and the synchronisation)."""
+    import torch
+    from tsmi import workloads_torch as WT
+    from tsmi.hip import HipKernel
+    lat, psi, v, params = WT.config2(n=n, device="cuda")
+    psi_re, psi_im = psi.real.contiguous(), psi.imag.contiguous()
+    e_re, e_im = WT.exp_potential(v, params["dt"], False)
+    out = {"lattice": [lat.dim_x, lat.dim_y], "calls": calls}
+    for prec in (32, 64):
+        k = HipKernel(lat, delta_t=params["dt"], mass=params["mass"], coupling=params["g"], precision=prec)
+        k.set_exp_potential_device(e_re.data_ptr(), e_im.data_ptr())
+        k.set_potential_device(v.data_ptr())
+        k.set_state_device(psi_re.data_ptr(), psi_im.data_ptr())
+        torch.cuda.synchronize()
+        for name, fn in (("energy", k.energy), ("expected_values", k.expected_values),
+                         ("squared_norm", k.squared_norm)):
+            fn()
+            t = time.perf_counter()
+            for _ in range(calls):
+                fn()
+            out[f"fp{prec}_{name}_ms_per_call"] = (time.perf_counter() - t) / calls * 1e3
+        k.close()
+    return out
+
+
 def pair_study(n: int = 4096, steps: int = 200):
     """Two-component real-time GPE with Rabi coupling on an n x n periodic
     lattice (two seeded Thomas-Fermi clouds, g_a = g_b = 50, g_ab = 40,
@@ -440,7 +468,7 @@ def main():
             calibrate_copy()
             return
         fn = {"precision_study": precision_study, "batch": batch_study, "adjoint": adjoint_study,
-              "pair": pair_study}[args.only_study]
+              "pair": pair_study, "observables": observables_study}[args.only_study]
         print(json.dumps({args.only_study: fn()}), flush=True)
         return
     from tsmi import workloads_torch as WT

@@ -285,7 +285,7 @@ int tsmi_energy(tsmi_ctx *ctx, double out[6]);
  * hipStream_t; NULL: the null stream) and not waited for.  Inner regions wider
  * than TSMI_NOALLOC_MAX_COLUMNS columns are rejected (TSMI_E_INVALID). */
 #define TSMI_NOALLOC_SCRATCH_BYTES (4096 * 14 * 8)
-#define TSMI_NOALLOC_MAX_COLUMNS (4096 * 256)
+#define TSMI_NOALLOC_MAX_COLUMNS (4096 * 240)
 int tsmi_energy_device_noalloc(const tsmi_desc *desc, const void *psi_re, const void *psi_im, int64_t pitch,
                                const double *v, double *out, void *stream);
 

@@ -78,8 +78,8 @@ def test_noalloc_rejects_bad_arguments():
                                         out.data_ptr(), None) == 1
     assert b"pitch" in L.tsmi_last_error()
     # an inner region wider than the scratch array covers (the check comes before any device work)
-    wide = _lib.Desc(4096 * 256 + 1, 2, 0, 0, 4, 4, 0, 64, 0, 0, 1e-3, 1.0, 0.0, 0.0, 0.1, 0.1, 1.0, 1.0, 0.0)
-    assert L.tsmi_energy_device_noalloc(C.byref(wide), buf.data_ptr(), buf.data_ptr(), 4096 * 256 + 1, None,
+    wide = _lib.Desc(4096 * 240 + 1, 2, 0, 0, 4, 4, 0, 64, 0, 0, 1e-3, 1.0, 0.0, 0.0, 0.1, 0.1, 1.0, 1.0, 0.0)
+    assert L.tsmi_energy_device_noalloc(C.byref(wide), buf.data_ptr(), buf.data_ptr(), 4096 * 240 + 1, None,
                                         out.data_ptr(), None) == 1
     assert b"wider" in L.tsmi_last_error()
     assert L.tsmi_energy_device_noalloc(None, None, None, 0, None, None, None) == 1

@@ -5,11 +5,12 @@
 // (src/solver.cpp:320-420, single component, angular velocity 0) and of
 // State::calculate_expected_values (src/model.cpp:408-455).
 //
-// Layout of the pass: a thread owns one column of the inner region and walks a
-// band of rows, keeping the five rows y-2..y+2 of its column in registers (the
-// y stencils), so every state value is loaded once per band; the x stencils
-// read the neighbouring columns, which the neighbouring lanes of the same load
-// bring into the cache.  Each thread sums 32 rows plainly and adds the chunk
+// Layout of the pass: a thread owns one column and walks a band of rows,
+// keeping the five rows y-2..y+2 of its column in registers (the y stencils)
+// and loading row y+3 one row ahead, so every state value is loaded once per
+// band; the x stencils come from the neighbouring lanes through DPP lane
+// shifts (a wave sums 60 columns, its two outer lanes on each side only supply
+// stencil values).  Each thread sums 32 rows plainly and adds the chunk
 // sums with Neumaier compensation; a wave combines its lanes in a fixed xor
 // tree, a block its four waves in order, and obs_finalize_kernel sums the block
 // partials with Neumaier compensation in block order -- the result does not
@@ -18,6 +19,8 @@
 // homogeneous in psi: degree 2, 4 for the interaction sum, 5 for LHY).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include "step_kernel.hpp"   // DPP lane shifts
 
 namespace tsmi {
 
@@ -40,10 +43,12 @@ enum ObsIndex {
 enum ObsKind { OBS_KIND_NORM = 0, OBS_KIND_ENERGY = 1, OBS_KIND_ALL = 2 };
 
 constexpr int kObsThreads = 256;
+constexpr int kObsWaveCols = 60;         // columns summed per wave (64 lanes, 2 stencil lanes each side)
+constexpr int kObsBlockCols = kObsThreads / 64 * kObsWaveCols;
 constexpr int kObsChunk = 32;            // rows summed plainly before a compensated add
 // Block partials a pass may write: the grid is (column blocks) x (row bands)
 // with at most this many blocks, so the inner region may be at most
-// kObsBlocksMax * kObsThreads columns wide.
+// kObsBlocksMax * kObsBlockCols columns wide.
 constexpr int kObsBlocksMax = 4096;
 constexpr int kObsOut = 16;              // obs_finalize_kernel's outputs
 
@@ -78,9 +83,13 @@ __global__ void __launch_bounds__(kObsThreads) observables_kernel(ObsParams<T> p
     double acc[NQ], s[NQ], c[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) acc[i] = s[i] = c[i] = 0.0;
-    const int x = p.x0 + (int)blockIdx.x * kObsThreads + (int)threadIdx.x;
-    const bool col = x < p.x1;
-    const int xs = col ? x : p.x0;                 // lanes past the region load a valid column
+    // lane l of a wave holds column x = base + l - 2; lanes 0, 1, 62, 63 only
+    // supply the x stencils of their neighbours (DPP lane shifts)
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)blockIdx.x * (kObsThreads / 64) + ((int)threadIdx.x >> 6);
+    const int x = p.x0 + wave * kObsWaveCols + lane - 2;
+    const bool col = lane >= 2 && lane < 62 && x < p.x1;
+    const int xs = min(max(x, 0), p.W - 1);        // every lane loads a column of the tile
     const int yb0 = p.y0 + (int)blockIdx.y * p.band;
     const int yb1 = min(p.y1, yb0 + p.band);
     const long P = p.pitch;
@@ -91,33 +100,43 @@ __global__ void __launch_bounds__(kObsThreads) observables_kernel(ObsParams<T> p
     // rows outside the tile read as row 0 or H-1: such values only meet cells
     // outside the stencil margin, whose sums do not use them
     auto row_of = [&](int y) { return (long)min(max(y, 0), p.H - 1) * P; };
-    double wr[5], wi[5];                           // rows y-2 .. y+2 of this column
-    if (KIND != OBS_KIND_NORM) {
+    // rows y-2 .. y+2 of this column, and row y+3 loaded one row ahead
+    double wr[5], wi[5], nr = 0.0, ni = 0.0, vn = 0.0;
+    if (KIND == OBS_KIND_NORM) {
+        const long o = row_of(yb0) + xs;
+        nr = (double)p.re[o];
+        ni = (double)p.im[o];
+    } else {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < 5; ++i) {
             const long o = row_of(yb0 - 2 + i) + xs;
             wr[i] = (double)p.re[o];
             wi[i] = (double)p.im[o];
         }
+        if (p.v) vn = p.v[row_of(yb0) + xs];
     }
     int chunk = 0;
     for (int y = yb0; y < yb1; ++y) {
-        const long k = (long)y * P + xs;
-        double pr, pi;
+        double pr, pi, vk = 0.0;
         if (KIND == OBS_KIND_NORM) {
-            pr = (double)p.re[k];
-            pi = (double)p.im[k];
+            pr = nr;
+            pi = ni;
+            const long o = row_of(y + 1) + xs;
+            nr = (double)p.re[o];
+            ni = (double)p.im[o];
         } else {
-            const long o = row_of(y + 2) + xs;
-            wr[4] = (double)p.re[o];
-            wi[4] = (double)p.im[o];
+            const long o = row_of(y + 3) + xs;
+            nr = (double)p.re[o];
+            ni = (double)p.im[o];
+            vk = vn;
+            if (p.v) vn = p.v[row_of(y + 1) + xs];
             pr = wr[2];
             pi = wi[2];
         }
         const double n2 = col ? pr * pr + pi * pi : 0.0;
         acc[OBS_N] += n2;
         if constexpr (KIND != OBS_KIND_NORM) {
-            if (p.v) acc[OBS_POT] += n2 * p.v[k];
+            acc[OBS_POT] += n2 * vk;
             acc[OBS_INT] += n2 * n2;
             if (LHY) acc[OBS_LHY] += n2 * n2 * sqrt(n2);
             if constexpr (KIND == OBS_KIND_ALL) {
@@ -127,13 +146,14 @@ __global__ void __launch_bounds__(kObsThreads) observables_kernel(ObsParams<T> p
                 acc[OBS_Y] += n2 * Y;
                 acc[OBS_YY] += n2 * Y * Y;
             }
+            // the x stencil from the neighbouring lanes (wave-uniform shifts)
+            const double r1 = shfl_from_next(pr), i1 = shfl_from_next(pi);
+            const double rr = shfl_from_next(r1), ir = shfl_from_next(i1);
+            const double l1r = shfl_from_prev(pr), l1i = shfl_from_prev(pi);
+            const double l2r = shfl_from_prev(l1r), l2i = shfl_from_prev(l1i);
             const bool row_margin = p.rows_free || (y >= p.ky0 && y < p.ky1);
             if (col_margin && row_margin) {
                 acc[OBS_NK] += n2;
-                const double rr = (double)p.re[k + 2], ir = (double)p.im[k + 2];
-                const double r1 = (double)p.re[k + 1], i1 = (double)p.im[k + 1];
-                const double l1r = (double)p.re[k - 1], l1i = (double)p.im[k - 1];
-                const double l2r = (double)p.re[k - 2], l2i = (double)p.im[k - 2];
                 const double lxr = c1 * rr + c2 * r1 + c2 * l1r + c1 * l2r + c3 * pr;
                 const double lxi = c1 * ir + c2 * i1 + c2 * l1i + c1 * l2i + c3 * pi;
                 acc[OBS_LAPX] += pr * lxr + pi * lxi;
@@ -166,6 +186,8 @@ __global__ void __launch_bounds__(kObsThreads) observables_kernel(ObsParams<T> p
                 wr[i] = wr[i + 1];
                 wi[i] = wi[i + 1];
             }
+            wr[4] = nr;
+            wi[4] = ni;
         }
         if (++chunk == kObsChunk) {
             chunk = 0;
@@ -178,13 +200,13 @@ __global__ void __launch_bounds__(kObsThreads) observables_kernel(ObsParams<T> p
     }
     // the thread's total, the wave's in a fixed xor tree, the block's in wave order
     __shared__ double red[kObsThreads / 64][NQ];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wv = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
         nsum(s[i], c[i], acc[i]);
         double t = s[i] + c[i];
         for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-        if (lane == 0) red[wave][i] = t;
+        if (lane == 0) red[wv][i] = t;
     }
     __syncthreads();
     if ((int)threadIdx.x < NQ) {
@@ -217,8 +239,20 @@ __global__ void __launch_bounds__(64 * OBS_COUNT) obs_finalize_kernel(const doub
     __shared__ double q[OBS_COUNT];
     const int lane = threadIdx.x & 63, i = threadIdx.x >> 6;
     if (i < f.nq) {
+        // sixteen loads per lane in flight before the first add; lanes past the
+        // end add zeros, which leave their sum and compensation as they are
+        constexpr int kBatch = 16;
         double a = 0.0, cc = 0.0;
-        for (int b = lane; b < f.blocks; b += 64) nsum(a, cc, partials[(long)b * OBS_COUNT + i]);
+        for (int b0 = 0; b0 < f.blocks; b0 += kBatch * 64) {
+            double v[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) {
+                const int b = b0 + j * 64 + lane;
+                v[j] = b < f.blocks ? partials[(long)b * OBS_COUNT + i] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) nsum(a, cc, v[j]);
+        }
         double t = a + cc;
         for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
         if (lane == 0) q[i] = t;

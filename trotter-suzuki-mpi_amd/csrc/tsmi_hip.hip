@@ -46,7 +46,7 @@ int fail(int code, const std::string &msg) {
 // returns on that device.
 static __device__ __align__(256) double scratch[tsmi::kObsBlocksMax * tsmi::OBS_COUNT];
 static_assert(sizeof(scratch) == TSMI_NOALLOC_SCRATCH_BYTES, "the header states the scratch size");
-static_assert(TSMI_NOALLOC_MAX_COLUMNS == tsmi::kObsBlocksMax * tsmi::kObsThreads, "the header states the limit");
+static_assert(TSMI_NOALLOC_MAX_COLUMNS == tsmi::kObsBlocksMax * tsmi::kObsBlockCols, "the header states the limit");
 
 namespace tsmi {
 
@@ -690,11 +690,11 @@ ObsGeom obs_geom(const tsmi_desc &d, int W, int H, int pitch) {
     return g;
 }
 
-// Grid of a pass: one block per 256 columns of the inner region and row bands
+// Grid of a pass: one block per kObsBlockCols columns of the inner region and row bands
 // of at least 16 rows, at most kObsBlocksMax blocks.  Returns false when the
-// inner region is wider than kObsBlocksMax * 256 columns.
+// inner region is wider than kObsBlocksMax * kObsBlockCols columns.
 bool obs_grid(const ObsGeom &g, dim3 *grid, int *band) {
-    const int cb = (g.x1 - g.x0 + kObsThreads - 1) / kObsThreads;
+    const int cb = (g.x1 - g.x0 + kObsBlockCols - 1) / kObsBlockCols;
     if (cb < 1 || cb > kObsBlocksMax) return false;
     const int rows = std::max(1, g.y1 - g.y0);
     const int rb_max = kObsBlocksMax / cb;
@@ -716,7 +716,7 @@ int launch_observables(const tsmi_desc &d, const ObsGeom &g, const T *re, const 
     int band = 0;
     if (!obs_grid(g, &grid, &band))
         return fail(TSMI_E_INVALID, std::string(who) + ": the inner region is wider than " +
-                                        std::to_string(kObsBlocksMax * kObsThreads) + " columns");
+                                        std::to_string(kObsBlocksMax * kObsBlockCols) + " columns");
     ObsParams<T> p;
     std::memset(&p, 0, sizeof(p));
     p.re = re; p.im = im; p.v = v; p.xc = xc; p.yc = yc;
