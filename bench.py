@@ -44,10 +44,9 @@ BYTES_PER_SU = {32: 24, 64: 48}   # psi read + psi write + E read (two planes) p
 # whether the launch advances one step or two
 KERNEL_BYTES_PER_SITE = {32: 20, 64: 40}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_8192_traffic.json")
-# the sources the step kernels are built from: a traffic record measured on
+# the sources of the step kernels' device code: a traffic record measured on
 # other sources is not reported
-KERNEL_SOURCES = ("trotter-suzuki-mpi_amd/csrc/step_kernel.hpp", "trotter-suzuki-mpi_amd/csrc/fused_kernel.hpp",
-                  "trotter-suzuki-mpi_amd/csrc/tsmi_hip.hip")
+KERNEL_SOURCES = ("trotter-suzuki-mpi_amd/csrc/step_kernel.hpp", "trotter-suzuki-mpi_amd/csrc/fused_kernel.hpp")
 ACCURACY_STEPS = 1000
 
 
