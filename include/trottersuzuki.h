@@ -12,9 +12,8 @@
  * through HipPairKernel.
  *
  * Outside this path (and rejected with std::runtime_error, as the reference's
- * my_abort does in a build without MPI): Lattice1D, cylindrical coordinates,
- * a rotating frame (angular_velocity != 0), kernel types other than
- * "hip"/"hip-fp32".
+ * my_abort does in a build without MPI): cylindrical coordinates, a rotating
+ * frame (angular_velocity != 0), kernel types other than "hip"/"hip-fp32".
  */
 #ifndef TSMI_TROTTERSUZUKI_H
 #define TSMI_TROTTERSUZUKI_H
@@ -44,6 +43,13 @@ public:
     int inner_end_x, inner_end_y;
     int mpi_coords[2], mpi_dims[2];
     virtual ~Lattice() {}
+};
+
+/** 1D chain (src/trottersuzuki.h:64-75, src/model.cpp:32-87): one row, no y
+ *  halo, delta_y = 1, length_y = 0.  Cartesian only. */
+class Lattice1D : public Lattice {
+public:
+    Lattice1D(int dim, double length, bool periodic_x_axis = false, string coordinate_system = "cartesian");
 };
 
 /** 2D lattice (src/trottersuzuki.h:83-117, src/model.cpp:103-158). */
@@ -103,9 +109,11 @@ protected:
     double norm2;
 };
 
-/** exp(i 2 pi (n_x x / L_x + n_y y / L_y)) (src/model.cpp:653-672). */
+/** exp(i 2 pi (n_x x / L_x + n_y y / L_y)) (src/model.cpp:640-672). */
 class ExponentialState : public State {
 public:
+    ExponentialState(Lattice1D *grid, int n_x = 1, double norm = 1, double phase = 0, double *p_real = 0,
+                     double *p_imag = 0);
     ExponentialState(Lattice2D *grid, int n_x = 1, int n_y = 1, double norm = 1, double phase = 0,
                      double *p_real = 0, double *p_imag = 0);
 
@@ -115,9 +123,11 @@ private:
     complex<double> exp_state(double x, double y);
 };
 
-/** Gaussian (src/model.cpp:692-714). */
+/** Gaussian (src/model.cpp:674-714). */
 class GaussianState : public State {
 public:
+    GaussianState(Lattice1D *grid, double omega_x, double mean_x = 0, double norm = 1, double phase = 0,
+                  double *p_real = 0, double *p_imag = 0);
     GaussianState(Lattice2D *grid, double omega_x, double omega_y = -1., double mean_x = 0, double mean_y = 0,
                   double norm = 1, double phase = 0, double *p_real = 0, double *p_imag = 0);
 
@@ -126,9 +136,12 @@ private:
     complex<double> gauss_state(double x, double y);
 };
 
-/** Product of sines (src/model.cpp:729-748). */
+/** Product of sines (src/model.cpp:716-748; on a Lattice1D n_y is 0, so the
+ *  state is zero, as in the reference). */
 class SinusoidState : public State {
 public:
+    SinusoidState(Lattice1D *grid, int n_x = 1, double norm = 1, double phase = 0, double *p_real = 0,
+                  double *p_imag = 0);
     SinusoidState(Lattice2D *grid, int n_x = 1, int n_y = 1, double norm = 1, double phase = 0,
                   double *p_real = 0, double *p_imag = 0);
 

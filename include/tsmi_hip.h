@@ -41,7 +41,7 @@ typedef struct tsmi_ctx tsmi_ctx;
 typedef struct {
     int32_t dim_x, dim_y;          /* inner lattice points (global_no_halo_dim_x/y)      */
     int32_t periodic_x, periodic_y;
-    int32_t halo_x, halo_y;        /* lattice halo widths; 4 on this path                */
+    int32_t halo_x, halo_y;        /* lattice halo widths: 4 (halo_y 0 for a one-row chain) */
     int32_t imag_time;             /* 0: real time, 1: imaginary time                    */
     int32_t precision;             /* 64: FP64 throughout; 32: FP32 bulk, FP64 reductions */
     int32_t tile_rows;             /* rows per wavefront segment; 0 chooses              */

@@ -56,7 +56,9 @@ int check_desc(const tsmi_desc &d, const char *who) {
     const std::string w(who);
     if (d.dim_x < 1 || d.dim_y < 1) return fail(TSMI_E_INVALID, w + ": lattice dimensions must be positive");
     if (d.precision != 32 && d.precision != 64) return fail(TSMI_E_INVALID, w + ": precision must be 32 or 64");
-    if (d.halo_x != 4 || d.halo_y != 4)
+    // a Lattice1D (one row, closed y) has no y halo (src/model.cpp:68-69)
+    const bool chain = d.dim_y == 1 && !d.periodic_y && d.halo_y == 0;
+    if (d.halo_x != 4 || (d.halo_y != 4 && !chain))
         return fail(TSMI_E_UNSUPPORTED, w + ": only the halo width 4 of a non-rotating lattice is supported");
     if (!(d.delta_x > 0) || !(d.delta_y > 0) || !(d.mass != 0))
         return fail(TSMI_E_INVALID, w + ": lattice spacing and mass must be non-zero");

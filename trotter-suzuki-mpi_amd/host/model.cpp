@@ -49,6 +49,40 @@ void map_lattice_to_coordinate_space(Lattice *grid, int x_in, int y_in, double *
 }
 
 // ---------------------------------------------------------------------------
+// Lattice1D (src/model.cpp:32-87): one row, no y halo, delta_y = 1, length_y = 0
+// ---------------------------------------------------------------------------
+Lattice1D::Lattice1D(int dim, double length, bool px, string cs) {
+    if (cs != "cartesian" && cs != "cylindrical")
+        throw std::runtime_error("The coordinate system you have chosen is not implemented.");
+    if (cs == "cylindrical") throw std::runtime_error("Cylindrical coordinates are outside the HIP path.");
+    if (dim < 1) throw std::runtime_error("Lattice dimensions must be positive.");
+    coordinate_system = cs;
+    length_x = length;
+    length_y = 0;
+    delta_x = length_x / double(dim);
+    delta_y = 1.0;
+    periods[0] = 0;
+    periods[1] = (int)px;
+    mpi_procs = 1;
+    mpi_rank = 0;
+    mpi_dims[0] = mpi_dims[1] = 1;
+    mpi_coords[0] = mpi_coords[1] = 0;
+    halo_x = kHalo;
+    halo_y = 0;
+    global_dim_x = dim + periods[1] * 2 * halo_x;
+    global_dim_y = 1;
+    global_no_halo_dim_x = dim;
+    global_no_halo_dim_y = 1;
+    borders(dim, halo_x, px, &start_x, &end_x, &inner_start_x, &inner_end_x);
+    dim_x = end_x - start_x;
+    start_y = 0;
+    end_y = 1;
+    inner_start_y = 0;
+    inner_end_y = 1;
+    dim_y = 1;
+}
+
+// ---------------------------------------------------------------------------
 // Lattice2D (src/model.cpp:89-158)
 // ---------------------------------------------------------------------------
 Lattice2D::Lattice2D(int dim, double length, bool px, bool py, double omega, string cs) {
@@ -305,8 +339,37 @@ void State::write_phase(string prefix) {
 }
 
 // ---------------------------------------------------------------------------
-// built-in states (src/model.cpp:653-748)
+// built-in states (src/model.cpp:640-748); on a Lattice1D the functions are
+// evaluated at y = 0 with n_y = 0 (and omega_y = 1 for the Gaussian)
 // ---------------------------------------------------------------------------
+namespace {
+template <typename F>
+void fill_row(Lattice *grid, double *re, double *im, F f) {
+    double xr, yr;
+    for (int x = 0; x < grid->dim_x; ++x) {
+        map_lattice_to_coordinate_space(grid, x, 0, &xr, &yr);
+        const complex<double> v = f(xr, 0.);
+        re[x] = real(v);
+        im[x] = imag(v);
+    }
+}
+}  // namespace
+
+ExponentialState::ExponentialState(Lattice1D *g, int nx, double nrm, double ph, double *re, double *im)
+    : State(g, 0, re, im), n_x(nx), n_y(0), norm(nrm), phase(ph) {
+    fill_row(grid, p_real, p_imag, [this](double x, double y) { return exp_state(x, y); });
+}
+
+GaussianState::GaussianState(Lattice1D *g, double ox, double mx, double nrm, double ph, double *re, double *im)
+    : State(g, 0, re, im), mean_x(mx), mean_y(0.), omega_x(ox), omega_y(1.), norm(nrm), phase(ph) {
+    fill_row(grid, p_real, p_imag, [this](double x, double y) { return gauss_state(x, y); });
+}
+
+SinusoidState::SinusoidState(Lattice1D *g, int nx, double nrm, double ph, double *re, double *im)
+    : State(g, 0, re, im), n_x(nx), n_y(0), norm(nrm), phase(ph) {
+    fill_row(grid, p_real, p_imag, [this](double x, double y) { return sinusoid_state(x, y); });
+}
+
 ExponentialState::ExponentialState(Lattice2D *g, int nx, int ny, double nrm, double ph, double *re, double *im)
     : State(g, 0, re, im), n_x(nx), n_y(ny), norm(nrm), phase(ph) {
     double xr, yr;

@@ -76,6 +76,10 @@ void Solver::set_exp_potential(double *re, int re_len, double *im, int im_len, i
     is_python = true;
     std::memcpy(external_pot_real[which], re, sizeof(double) * re_len);
     std::memcpy(external_pot_imag[which], im, sizeof(double) * im_len);
+    // The reference's CPU kernel keeps pointers to these arrays and reads the
+    // new table at its next step (src/cpukernel.cpp:252-253); the device
+    // kernel holds its own copy, so the table is handed over here.
+    if (kernel) kernel->update_potential(external_pot_real[which], external_pot_imag[which], which);
 }
 
 // the kernel factory (src/solver.cpp:118-148) for the HIP kernel types
