@@ -106,9 +106,6 @@ __device__ __forceinline__ void unroll_beats(F &&f, std::integer_sequence<int, K
 #ifndef TSMI_STREAM_AHEAD
 #define TSMI_STREAM_AHEAD 4
 #endif
-#ifndef TSMI_STREAM_LOADS_FIRST
-#define TSMI_STREAM_LOADS_FIRST 0
-#endif
 #ifndef TSMI_STREAM
 #define TSMI_STREAM 1
 #endif
@@ -173,19 +170,10 @@ __device__ __forceinline__ void step2_stream_f32(__amdgpu_buffer_rsrc_t r_in_re,
                 Row<float, 2> &out = s[(k + S - 8) % S];
                 const int y = j - 8;
                 const unsigned off = (y >= out_lo && y < out_hi) ? v_st + row_off(y) : kOffNone;
-                if (TSMI_STREAM_LOADS_FIRST) {
-                    const float o0 = out.re[0], o1 = out.re[1], o2 = out.im[0], o3 = out.im[1];
-                    load_phase(j + D - 2, e[(k + D - 2) % S]);
-                    load_state(j + D, out);
-                    const float ore[2] = {o0, o1}, oim[2] = {o2, o3};
-                    buf_store<float, 2>(r_out_re, off, 0u, ore);
-                    buf_store<float, 2>(r_out_im, off, 0u, oim);
-                } else {
-                    buf_store<float, 2>(r_out_re, off, 0u, out.re);
-                    buf_store<float, 2>(r_out_im, off, 0u, out.im);
-                    load_state(j + D, out);
-                    load_phase(j + D - 2, e[(k + D - 2) % S]);
-                }
+                buf_store<float, 2>(r_out_re, off, 0u, out.re);
+                buf_store<float, 2>(r_out_im, off, 0u, out.im);
+                load_state(j + D, out);
+                load_phase(j + D - 2, e[(k + D - 2) % S]);
             },
             std::make_integer_sequence<int, S>());
     }

@@ -78,10 +78,10 @@ struct StepParams {
 // 548-560 divides by it).  Lane-strided Neumaier sums in wave order, then an
 // xor butterfly: partner lanes add the same two operands, so every lane of the
 // wave returns the same value, and every caller the same bits.  The partials
-// are read sixteen per lane at a time (all loads issued before the first wait);
+// are read eight per lane at a time (all loads issued before the first wait);
 // a lane past the end adds zeros, which leave its sum and compensation as they are.
 __device__ __forceinline__ double renorm_divisor(const double *part, int n, double dxdy, double target, int lane) {
-    constexpr int kBatch = 16;
+    constexpr int kBatch = 8;
     double a = 0.0, c = 0.0;
     for (int i0 = 0; i0 < n; i0 += kBatch * kWave) {
         double x[kBatch];
@@ -670,13 +670,28 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     // Writes the row leaving the pipeline.
     // Imaginary time: adds the inner cells of an output row to the sum (the
     // lane's few cells of the row summed first, then one compensated step).
+    // FP32: the lane's cells of the row are summed in FP32 (at most 2 VEC
+    // products, a relative error below 1e-6 that the next renormalisation
+    // removes), then join the FP64 sum.
     auto accumulate = [&](int y, const Row<T, VEC> &o4) {
         if (IMAG && y >= p.iy0 && y < p.iy1) {
             double v = 0.0;
+            if constexpr (sizeof(T) == 4) {
+                float v32 = 0.0f;
 #pragma unroll
-            for (int k = 0; k < VEC; ++k) {
-                const double vr = double(o4.re[k]), vi = double(o4.im[k]);
-                if (inner_mask & (1u << k)) v += vr * vr + vi * vi;
+                for (int k = 0; k < VEC; ++k) {
+                    const float w = (inner_mask & (1u << k)) ? o4.re[k] : 0.0f;
+                    const float z = (inner_mask & (1u << k)) ? o4.im[k] : 0.0f;
+                    v32 = fmaf(w, w, v32);
+                    v32 = fmaf(z, z, v32);
+                }
+                v = double(v32);
+            } else {
+#pragma unroll
+                for (int k = 0; k < VEC; ++k) {
+                    const double vr = double(o4.re[k]), vi = double(o4.im[k]);
+                    if (inner_mask & (1u << k)) v += vr * vr + vi * vi;
+                }
             }
             const double t2 = acc + v;
             comp += (fabs(acc) >= fabs(v)) ? (acc - t2) + v : (v - t2) + acc;
@@ -689,10 +704,10 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         buf_store<T, VEC>(r_out_im, v_st, so, o4.im);
         accumulate(y, o4);
     };
-    // The emit of the steady state, for rows below y_out_hi: a row before
-    // y_out_lo is dropped by a store offset past the buffer, not by a branch.
+    // The emit of the fast form: a row outside [y_out_lo, y_out_hi) is dropped
+    // by a store offset past the buffer, not by a branch.
     auto emit_any = [&](int y, const Row<T, VEC> &o4) {
-        const bool out = y >= y_out_lo;
+        const bool out = y >= y_out_lo && y < y_out_hi;
         const unsigned off = out ? v_st + (unsigned)(y - y_lo) * row_bytes : kOffNone;
         buf_store<T, VEC>(r_out_re, off, 0u, o4.re);
         buf_store<T, VEC>(r_out_im, off, 0u, o4.im);
@@ -832,7 +847,18 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         load_psi_any(t + 7, inD);
         load_e_any(t + 5, eB);
     };
-    if (interior) {
+    // Every beat in the fast form when the segment's four apron rows above and
+    // below lie inside the lattice: the zero rows the pipeline starts with and
+    // the zero rows past the segment then meet apron rows only.  A pair with a
+    // wrong partner leaves a wrong value in an apron row; one vertical pass
+    // moves it at most one row, four a step, so it never reaches an output row
+    // (the 4-column strip aprons rest on the same bound).  The row offsets of
+    // the last beats stay below 2^31.
+    const bool all_fast = interior && y_out_lo >= kApron && y_out_hi + kApron <= p.H &&
+                          (unsigned long)(y_hi - y_lo + 12) * row_bytes < 0x80000000ul;
+    if (all_fast) {
+        for (; t < t_end; t += 4) group_fast(t);
+    } else if (interior) {
         // rows y_lo .. y_lo+3: the pipeline fills
         for (; t < t_end && t < y_lo + 4; t += 4) group_general(t);
         for (; t + 3 < y_hi; t += 4) group_fast(t);

@@ -115,7 +115,9 @@ constexpr int kBlock = 256;          // 4 wavefronts per workgroup
 // (FP64 uses 2: one 16-byte piece per plane).  Compile-time tuning constants,
 // set from measurements at 8192^2: the fused kernel with 4 columns per lane
 // needs more registers than one wave per SIMD allows without spills, with 2 it
-// runs three waves per SIMD.
+// runs three waves per SIMD.  The imaginary-time step (small lattices, a
+// wave's row latency bounds it) uses 2: at 2048^2, 20-row segments, 26.4 us
+// per step against 29.0 with 4 (256^2: 16.0 against 20.3).
 #ifndef TSMI_FP32_VEC
 #define TSMI_FP32_VEC 4
 #endif
@@ -123,7 +125,7 @@ constexpr int kBlock = 256;          // 4 wavefronts per workgroup
 #define TSMI_FP32_VEC2 2
 #endif
 #ifndef TSMI_FP32_VEC_IMAG
-#define TSMI_FP32_VEC_IMAG 4
+#define TSMI_FP32_VEC_IMAG 2
 #endif
 constexpr int kVec32 = TSMI_FP32_VEC;
 constexpr int kVec32I = TSMI_FP32_VEC_IMAG;   // the imaginary-time single step
@@ -498,12 +500,12 @@ hipError_t occupancy_blocks(int precision, bool imag, bool lhy, int *blocks) {
 // Rows per segment of the one-step pass.  One round of resident wavefronts: as
 // many row segments per strip as the device holds at this kernel's occupancy,
 // so no partial second round idles most of the GPU; on small lattices at least
-// 24 rows, below which the 4-row aprons and the per-wave partial sums cost more
-// than the extra waves bring -- unless 24-row segments leave SIMDs without a
+// 20 rows, below which the 4-row aprons and the per-wave partial sums cost more
+// than the extra waves bring -- unless 20-row segments leave SIMDs without a
 // wave, where a wave's row latency bounds the step and 12-row segments are
 // faster (measured, imaginary time: 1024^2 FP32 28.9 -> 22.5 us, FP64 30.3 ->
-// 23.2; 2048^2 FP32 33.7 -> 31.3; FP64 at 2048^2, one wave per SIMD at 24 rows,
-// 50.0 vs 52.3 with 12).
+// 23.2; at 2048^2 20-row segments against 24: FP32 26.4 vs 28.0, FP64 44.0 vs
+// 45.8, and against 12: 27.3 and 52.2).
 int plan_single(const tsmi_desc &d, int H, long long pitch, int strips) {
     long resident = 0, simds = 0;
     int dev = 0, cus = 0, blocks_per_cu = 0;
@@ -515,7 +517,7 @@ int plan_single(const tsmi_desc &d, int H, long long pitch, int strips) {
         simds = 4L * cus;
     }
     (void)hipGetLastError();
-    return plan_seg_rows(H, strips, resident, simds, 24, true, d.tile_rows, pitch, 2 * kApron);
+    return plan_seg_rows(H, strips, resident, simds, 20, true, d.tile_rows, pitch, 2 * kApron);
 }
 
 template <typename T, bool IMAG, int VEC, bool LHY, bool PH>
