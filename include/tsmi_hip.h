@@ -125,8 +125,11 @@ int tsmi_run_steps(tsmi_ctx *ctx, int32_t n_steps, int32_t exchange_last);
 int tsmi_step(tsmi_ctx *ctx, int32_t exchange);
 
 /* Gradient of a loss of the final state with respect to the potential.
- * No reference counterpart (the reference evolves forward only).  Real time,
- * closed axes and g = g_LHY = 0 only (TSMI_E_UNSUPPORTED otherwise).
+ * No reference counterpart (the reference evolves forward only).  Real time
+ * only (TSMI_E_UNSUPPORTED otherwise); any g and g_LHY; closed or periodic axes,
+ * except a periodic x axis with a closed y axis of a 2D lattice (the reference
+ * refreshes the x halo of only H - 2*halo_y rows there) and a periodic axis
+ * whose inner extent is below twice the halo.
  *
  * tsmi_evolve_checkpointed: tsmi_evolve(n_steps) that also keeps the state
  * every `every` steps on the device (ceil(n_steps/every) copies of the tile).
@@ -137,9 +140,13 @@ int tsmi_step(tsmi_ctx *ctx, int32_t exchange);
  * every tile cell for E = exp(-i dt V) (W*H FP64); g0_re/g0_im (optional, may
  * be NULL) = dL/dRe psi_0 and dL/dIm psi_0.  Each reverse step is the step at
  * -dt applied to the adjoint and to the state together, the state re-anchored
- * on the checkpoints; the FP64 gradient accumulates dt Im(conj(mu) psit) of the
- * two mid-step states.  The host form syncs; the device form is asynchronous on
- * the context stream.  The state of the context is left at psi_N. */
+ * on the checkpoints; the FP64 gradient accumulates dt Im(conj(mu) y) of the
+ * two mid-step states.  On periodic axes the lattice is the periodic one: the
+ * halo entries of g are taken as acting on the inner cells they mirror, and
+ * grad_v and g0 hold, on each inner cell, the derivative with respect to the
+ * value the cell shares with its halo copies (zero on the halo).  The host form
+ * syncs; the device form is asynchronous on the context stream.  The state of
+ * the context is left at psi_N. */
 int tsmi_evolve_checkpointed(tsmi_ctx *ctx, int32_t n_steps, int32_t every);
 int tsmi_adjoint(tsmi_ctx *ctx, const double *g_re, const double *g_im, double *grad_v,
                  double *g0_re, double *g0_im);

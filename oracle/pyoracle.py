@@ -55,6 +55,7 @@ def lib():
         L.ts_oracle_adjoint.argtypes = [C.POINTER(Desc), C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double,
                                         _dp, _dp, _dp, _dp, _dp]
         L.ts_oracle_adjoint.restype = C.c_int
+        L.ts_oracle_halo_exchange_adjoint.argtypes = [C.POINTER(Desc), _dp, _dp]
         _lib = L
     return _lib
 
@@ -137,7 +138,8 @@ def expected_values(wl, psi):
 
 
 def adjoint(wl, g_final: np.ndarray, steps: int | None = None, every: int = 100):
-    """Oracle gradient of L(psi_N) with respect to V (FP64 discrete adjoint).
+    """Oracle gradient of L(psi_N) with respect to V (FP64 discrete adjoint of
+    the real-time evolve, every tile entry its own variable).
 
     g_final = dL/dRe psi_N + i dL/dIm psi_N on the tile.  Returns
     (dL/dV, dL/dRe psi_0 + i dL/dIm psi_0)."""
@@ -156,6 +158,26 @@ def adjoint(wl, g_final: np.ndarray, steps: int | None = None, every: int = 100)
     if rc != 0:
         raise ValueError(f"ts_oracle_adjoint: unsupported case (code {rc})")
     return gv, g0r + 1j * g0i
+
+
+def halo_exchange(wl, field: np.ndarray) -> np.ndarray:
+    """The periodic halo refresh of a real or complex tile field (a copy)."""
+    d = make_desc(wl, norm_target=1.0)
+    re = np.ascontiguousarray(np.real(field), dtype=np.float64).copy()
+    im = np.ascontiguousarray(np.imag(field), dtype=np.float64).copy()
+    lib().ts_oracle_halo_exchange(C.byref(d), _p(re), _p(im))
+    return re + 1j * im if np.iscomplexobj(field) else re
+
+
+def fold_halo(wl, field: np.ndarray) -> np.ndarray:
+    """The transpose of the halo refresh (a copy): every halo entry added onto
+    the cell it mirrors and cleared -- derivatives with respect to tile entries
+    become derivatives with respect to a periodic field."""
+    d = make_desc(wl, norm_target=1.0)
+    re = np.ascontiguousarray(np.real(field), dtype=np.float64).copy()
+    im = np.ascontiguousarray(np.imag(field), dtype=np.float64).copy()
+    lib().ts_oracle_halo_exchange_adjoint(C.byref(d), _p(re), _p(im))
+    return re + 1j * im if np.iscomplexobj(field) else re
 
 
 # ---------------------------------------------------------------------------

@@ -316,22 +316,30 @@ void ts_oracle_expected_values(const ts_oracle_desc *d, const double *re, const 
 }
 
 /* ---------------------------------------------------------------------------
- * Discrete adjoint of the real-time step with respect to the potential.
+ * Discrete adjoint of the real-time evolution with respect to the potential.
  *
  * Not a reference function: the reference has no gradient.  It is the exact
- * derivative of ts_oracle_evolve (closed axes, real time, g = g_LHY = 0), so it
- * is pinned by central finite differences of that forward (and therefore of the
- * reference solver, which the forward matches bit for bit).
+ * derivative of ts_oracle_evolve in real time (any boundaries, any g and
+ * g_LHY) with respect to every tile entry of V and of psi_0, so it is pinned by
+ * central finite differences of that forward (and therefore of the reference
+ * solver, which the forward matches bit for bit).
  *
- * With psi_{n+1} = A P A' psi_n, A' = the first four passes V0 H0 V1 H1, P the
- * multiplication by E = exp(-i dt V) and A the last four, and the adjoint state
- * G = dL/dRe psi + i dL/dIm psi:
- *     mu_n = A^H G_{n+1},  dL/dV_j += dt Im(conj(mu_j) E_j chi_j),  chi_n = A' psi_n,
- *     G_n = A'^H conj(E) mu_n.
- * A real-time pass [[a, ib], [ib, a]] is symmetric, so its adjoint is the same
- * pass with b -> -b; the passes of A^H run in the order of A' and vice versa.
- * Forward states are kept every `every` steps and each segment is recomputed
- * from its checkpoint before its reverse sweep.
+ * Step i of the forward is psi_{i+1} = X_i A N(A' psi_i): A' = the first four
+ * passes V0 H0 V1 H1, N the potential stage, A the last four, X_i the halo
+ * exchange (after every step but the last).  With the adjoint state
+ * G = dL/dRe psi + i dL/dIm psi, one reverse step is
+ *     G <- X_i^T G,  mu = A^H G,  G <- A'^H N^T mu,
+ * where a real-time pass [[a, ib], [ib, a]] is symmetric, so its adjoint is the
+ * same pass with b -> -b and the passes of A^H run in the order of A'.  At a
+ * cell with chi = A' psi_i, s = |chi|^2, phi = c s + c_LHY s^(3/2) and
+ * y = N(chi) = E chi e^(-i phi) (the forward's own arithmetic):
+ *     dL/dV += dt Im(conj(mu) y),
+ *     N^T mu = conj(E e^(-i phi)) mu + 2 phi'(s) Im(conj(mu) y) chi,
+ * phi'(s) = c + 1.5 c_LHY s^(1/2); with c = c_LHY = 0 this is conj(E) mu.  The
+ * exchange is a sequence of copies dst <- src; the adjoint of each adds dst
+ * into src and clears dst, in reverse order.  Forward states are kept every
+ * `every` steps and each segment is recomputed from its checkpoint before its
+ * reverse sweep.
  * ------------------------------------------------------------------------- */
 #include <stdlib.h>
 #include <string.h>
@@ -352,13 +360,67 @@ static void second_half(const ts_oracle_desc *d, double sgn, double *re, double 
     if (vert) ts_oracle_pass_vertical(d, 0, d->aV, sgn * d->bV, re, im);
 }
 
+/* Adjoint of copy2d: every element's copy undone in reverse order. */
+static void copy2d_adjoint(double *p, size_t dst, size_t src, size_t stride, size_t width, size_t height) {
+    for (size_t i = height; i-- > 0;)
+        for (size_t j = width; j-- > 0;) {
+            p[src + i * stride + j] += p[dst + i * stride + j];
+            p[dst + i * stride + j] = 0.;
+        }
+}
+
+void ts_oracle_halo_exchange_adjoint(const ts_oracle_desc *d, double *re, double *im) {
+    /* the copies of ts_oracle_halo_exchange, last first */
+    const size_t W = (size_t)d->W, H = (size_t)d->H;
+    const size_t hx = (size_t)d->halo_x, hy = (size_t)d->halo_y;
+    if (d->periodic_y) {
+        const size_t off = (H - hy) * W;
+        copy2d_adjoint(im, off, hy * W, W, W, hy);
+        copy2d_adjoint(re, off, hy * W, W, W, hy);
+        copy2d_adjoint(im, 0, off - hy * W, W, W, hy);
+        copy2d_adjoint(re, 0, off - hy * W, W, W, hy);
+    }
+    if (d->periodic_x) {
+        const size_t iy0 = d->periodic_y ? hy : 0;
+        const size_t rows = H > 2 * hy ? H - 2 * hy : 0;
+        const size_t off = iy0 * W;
+        copy2d_adjoint(im, off + W - hx, off + hx, W, hx, rows);
+        copy2d_adjoint(re, off + W - hx, off + hx, W, hx, rows);
+        copy2d_adjoint(im, off, off + W - 2 * hx, W, hx, rows);
+        copy2d_adjoint(re, off, off + W - 2 * hx, W, hx, rows);
+    }
+}
+
+/* N^T at every cell (see above); chi is the forward's pre-potential state. */
+static void potential_adjoint(const ts_oracle_desc *d, const double *chr, const double *chi_i,
+                              const double *e_re, const double *e_im, double dt,
+                              double *gr, double *gi, double *grad_v) {
+    const size_t n = (size_t)d->W * (size_t)d->H;
+    const double c = d->c_nl, cl = d->c_lhy;
+    for (size_t k = 0; k < n; ++k) {
+        const double cr = chr[k], ci = chi_i[k];
+        const double er = e_re[k], ei = e_im[k];
+        const double n2 = cr * cr + ci * ci;
+        const double ang = c * n2 + cl * (n2 * sqrt(n2));
+        const double cs = cos(ang), sn = sin(ang);
+        const double r1 = er * cr - ei * ci, i1 = er * ci + ei * cr;   /* E chi */
+        const double yr = cs * r1 + sn * i1, yi = cs * i1 - sn * r1;    /* y */
+        const double mr = gr[k], mi = gi[k];
+        const double z = mr * yi - mi * yr;                             /* Im(conj(mu) y) */
+        grad_v[k] += dt * z;
+        const double ar = er * mr + ei * mi, ai = er * mi - ei * mr;    /* conj(E) mu */
+        const double f = 2. * (c + 1.5 * cl * sqrt(n2)) * z;
+        gr[k] = (cs * ar - sn * ai) + f * cr;
+        gi[k] = (sn * ar + cs * ai) + f * ci;
+    }
+}
+
 int ts_oracle_adjoint(const ts_oracle_desc *d, int n_steps, int every,
                       const double *psi_re, const double *psi_im,
                       const double *e_re, const double *e_im, double dt,
                       const double *g_re, const double *g_im,
                       double *grad_v, double *g0_re, double *g0_im) {
-    if (d->imag_time || d->periodic_x || d->periodic_y || d->c_nl != 0. || d->c_lhy != 0. ||
-        n_steps < 0 || every < 1)
+    if (d->imag_time || n_steps < 0 || every < 1)
         return -1;
     const size_t n = (size_t)d->W * (size_t)d->H;
     const int n_ck = n_steps > 0 ? (n_steps - 1) / every + 1 : 0;
@@ -377,6 +439,7 @@ int ts_oracle_adjoint(const ts_oracle_desc *d, int n_steps, int every,
     for (int i = 0; i < n_steps; ++i) {
         if (i % every == 0) memcpy(ck + 2 * n * (size_t)(i / every), st, sizeof(double) * 2 * n);
         ts_oracle_full_step(d, st, st + n, e_re, e_im);
+        if (i != n_steps - 1) ts_oracle_halo_exchange(d, st, st + n);
     }
     memcpy(gg, g_re, sizeof(double) * n);
     memcpy(gg + n, g_im, sizeof(double) * n);
@@ -385,26 +448,20 @@ int ts_oracle_adjoint(const ts_oracle_desc *d, int n_steps, int every,
     for (int seg = n_ck - 1; seg >= 0; --seg) {
         const int s0 = seg * every;
         const int s1 = s0 + every < n_steps ? s0 + every : n_steps;
-        /* recompute chi_n = A' psi_n over the segment */
+        /* recompute chi_i = A' psi_i over the segment */
         memcpy(st, ck + 2 * n * (size_t)seg, sizeof(double) * 2 * n);
         for (int i = s0; i < s1; ++i) {
             first_half(d, 1., st, st + n);
             memcpy(chi + 2 * n * (size_t)(i - s0), st, sizeof(double) * 2 * n);
             ts_oracle_pass_potential(d, st, st + n, e_re, e_im);
             second_half(d, 1., st, st + n);
+            if (i != n_steps - 1) ts_oracle_halo_exchange(d, st, st + n);
         }
         for (int i = s1 - 1; i >= s0; --i) {
             const double *cr = chi + 2 * n * (size_t)(i - s0), *ci = cr + n;
+            if (i != n_steps - 1) ts_oracle_halo_exchange_adjoint(d, gr, gi);
             first_half(d, -1., gr, gi);                      /* mu = A^H G */
-            for (size_t k = 0; k < n; ++k) {
-                const double er = e_re[k], ei = e_im[k];
-                const double pr = er * cr[k] - ei * ci[k];   /* E chi */
-                const double pi = er * ci[k] + ei * cr[k];
-                grad_v[k] += dt * (gr[k] * pi - gi[k] * pr);
-                const double r = gr[k], m = gi[k];           /* conj(E) mu */
-                gr[k] = er * r + ei * m;
-                gi[k] = er * m - ei * r;
-            }
+            potential_adjoint(d, cr, ci, e_re, e_im, dt, gr, gi, grad_v);
             second_half(d, -1., gr, gi);                     /* A'^H */
         }
     }

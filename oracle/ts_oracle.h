@@ -89,18 +89,26 @@ void ts_oracle_expected_values(const ts_oracle_desc *d, const double *re, const 
                                const double *x_coord, const double *y_coord, double out[10]);
 
 /* Gradient of a loss L(psi_N) with respect to the potential V on every tile
- * cell, through n_steps real-time steps of ts_oracle_evolve (closed axes,
- * g = g_LHY = 0): the discrete adjoint, not a reference function.
+ * cell, through n_steps real-time steps of ts_oracle_evolve (any boundaries,
+ * g and g_LHY included): the discrete adjoint, not a reference function.  Each
+ * tile entry of V and of psi_0 is its own variable (a halo entry included).
  * g_re/g_im = dL/dRe psi_N, dL/dIm psi_N; E = exp(-i dt V) as for the forward.
  * Out: grad_v[W*H] = dL/dV; g0_re/g0_im (may be NULL) = dL/dRe psi_0, dL/dIm psi_0.
  * Checkpoints every `every` steps, each segment recomputed forward.
- * Returns 0, -1 for an unsupported descriptor, -2 when out of memory.
- * Pinned: central finite differences of ts_oracle_evolve (tests/test_adjoint.py). */
+ * Returns 0, -1 for an unsupported descriptor (imaginary time), -2 when out of
+ * memory.  Pinned: central finite differences of ts_oracle_evolve
+ * (tests/test_adjoint_oracle.py). */
 int ts_oracle_adjoint(const ts_oracle_desc *d, int n_steps, int every,
                       const double *psi_re, const double *psi_im,
                       const double *e_re, const double *e_im, double dt,
                       const double *g_re, const double *g_im,
                       double *grad_v, double *g0_re, double *g0_im);
+
+/* Adjoint (transpose) of ts_oracle_halo_exchange, in place: every halo entry is
+ * added onto the cell it copies and cleared.  On a field of derivatives with
+ * respect to tile entries it gives the derivatives with respect to the values
+ * shared by a cell and its halo copies (a periodic field). */
+void ts_oracle_halo_exchange_adjoint(const ts_oracle_desc *d, double *re, double *im);
 
 
 /* ---------------------------------------------------------------------------

@@ -1,6 +1,8 @@
 """The HIP reverse sweep (tsmi_evolve_checkpointed + tsmi_adjoint) against the
 oracle's FP64 discrete adjoint, which is pinned by central finite differences
-(tests/test_adjoint_oracle.py), and the torch.autograd bridge.
+(tests/test_adjoint_oracle.py), and the torch.autograd bridge: closed axes with
+and without the nonlinear and LHY terms, periodic axes (the gradient of the
+periodic fields: the oracle's tile gradient folded onto the inner cells).
 
 Bars: FP64 within 1e-10 of the oracle (max-norm, relative to the largest
 gradient entry); FP32 within 1e-4 (the FP32 adjoint target)."""
@@ -13,8 +15,8 @@ from tsmi.losses import infidelity
 pytestmark = pytest.mark.gpu
 
 
-def case(n_x, n_y, steps, dt=2e-3, length=12.0, seed=W.SEED_BASE + 91):
-    wl = W.small_case(n_x, n_y, False, False, steps=steps, dt=dt, length=length, seed=seed)
+def case(n_x, n_y, steps, dt=2e-3, length=12.0, seed=W.SEED_BASE + 91, g=0.0, g_lhy=0.0, px=False, py=False):
+    wl = W.small_case(n_x, n_y, px, py, steps=steps, dt=dt, length=length, seed=seed, g=g, g_lhy=g_lhy)
     lat = wl.lattice
     tx, ty = W.uniform(seed + 1, 2, lo=-1.5, hi=1.5)
     target = W.normalise(lat, W.gaussian_packet(lat, tx, ty, 0.0, 0.0, width=1.5))
@@ -25,10 +27,10 @@ def rel_max(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-def hip_adjoint(wl, target, oracle, precision, every, tile_rows=0):
+def hip_adjoint(wl, target, oracle, precision, every, tile_rows=0, sweeps=0):
     from tsmi.hip import kernel_for
     e = oracle.exp_potential(wl.v, wl.dt, False)
-    k = kernel_for(wl, precision=precision, tile_rows=tile_rows, e_planes=e)
+    k = kernel_for(wl, precision=precision, tile_rows=tile_rows, e_planes=e, sweeps=sweeps)
     try:
         k.evolve_checkpointed(wl.steps, every)
         psi = k.sample()
@@ -69,6 +71,56 @@ def test_adjoint_fp32_within_tolerance(c, oracle):
     assert rel_max(g0, ref_0) < 1e-4
 
 
+NL_CASES = [
+    dict(n_x=96, n_y=72, steps=80, every=20, g=40.0, g_lhy=0.0),
+    dict(n_x=601, n_y=75, steps=40, every=7, g=25.0, g_lhy=12.0),
+]
+
+
+@pytest.mark.parametrize("c", NL_CASES, ids=lambda c: f"{c['n_x']}x{c['n_y']}_g{c['g']:g}_lhy{c['g_lhy']:g}")
+@pytest.mark.parametrize("precision", [64, 32])
+def test_adjoint_nonlinear_matches_oracle(c, precision, oracle):
+    """g and g_LHY: the reverse step undoes the density-dependent phase and
+    pulls the adjoint through it (the 2 phi'(s) Im(conj(mu) y) x term)."""
+    wl, target = case(c["n_x"], c["n_y"], c["steps"], g=c["g"], g_lhy=c["g_lhy"])
+    _, g = infidelity(wl.lattice, target, oracle.evolve(wl))
+    ref_v, ref_0 = oracle.adjoint(wl, g, every=10)
+    _, gv, g0 = hip_adjoint(wl, target, oracle, precision, c["every"])
+    bar = 1e-10 if precision == 64 else 1e-4
+    assert rel_max(gv, ref_v) < bar
+    assert rel_max(g0, ref_0) < bar
+
+
+def periodic_case(oracle, px, py, g=30.0, steps=60, n_x=72, n_y=56):
+    """A packet crossing the periodic edges; V and psi_0 with refreshed halos
+    (periodic fields, as the reference's coordinate mapping makes them)."""
+    import dataclasses
+    wl, target = case(n_x, n_y, steps, dt=2e-3, length=8.0, g=g, px=px, py=py)
+    wl = dataclasses.replace(wl, v=oracle.halo_exchange(wl, wl.v), psi=oracle.halo_exchange(wl, wl.psi))
+    return wl, target
+
+
+@pytest.mark.parametrize("px,py,sweeps", [(True, True, 0), (True, True, 1), (False, True, 0)],
+                         ids=["xy_halo8", "xy_single", "y"])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_adjoint_periodic_matches_folded_oracle(px, py, sweeps, precision, oracle):
+    """Periodic axes: the library's gradient of the periodic fields against the
+    oracle's exact tile gradient folded onto the inner cells (fold_halo, pinned
+    against differences along periodic perturbations); zero on the halo."""
+    wl, target = periodic_case(oracle, px, py)
+    lat = wl.lattice
+    _, g = infidelity(lat, target, oracle.evolve(wl))
+    ref_v, ref_0 = oracle.adjoint(wl, g, every=10)
+    ref_v, ref_0 = oracle.fold_halo(wl, ref_v), oracle.fold_halo(wl, ref_0)
+    _, gv, g0 = hip_adjoint(wl, target, oracle, precision, 13, sweeps=sweeps)
+    halo = np.ones((lat.H, lat.W), bool)
+    halo[lat.inner_y, lat.inner_x] = False
+    assert not gv[halo].any() and not g0[halo].any()
+    bar = 1e-10 if precision == 64 else 1e-4
+    assert rel_max(gv, ref_v) < bar
+    assert rel_max(g0, ref_0) < bar
+
+
 def test_adjoint_segment_heights_agree(oracle):
     wl, target = case(120, 90, 30)
     _, a, _ = hip_adjoint(wl, target, oracle, 64, 10, tile_rows=8)
@@ -79,13 +131,14 @@ def test_adjoint_segment_heights_agree(oracle):
 def test_adjoint_errors():
     from tsmi import _lib
     from tsmi.hip import kernel_for
-    wl = W.small_case(32, 32, True, False, steps=3)
-    k = kernel_for(wl)
-    try:
-        with pytest.raises(_lib.TsmiError, match="closed"):
-            k.evolve_checkpointed(3, 1)
-    finally:
-        k.close()
+    for wl in (W.small_case(32, 32, False, False, steps=3, imag_time=True),
+               W.small_case(32, 32, True, False, steps=3)):      # x periodic, closed y: rows left unrefreshed
+        k = kernel_for(wl)
+        try:
+            with pytest.raises(_lib.TsmiError, match="real time"):
+                k.evolve_checkpointed(3, 1)
+        finally:
+            k.close()
     wl = W.small_case(32, 32, False, False, steps=3)
     k = kernel_for(wl)
     try:
@@ -126,25 +179,27 @@ def test_torch_autograd_matches_oracle(oracle):
         k.close()
 
 
-def test_energy_loss_gradient_against_oracle_fd(oracle):
+@pytest.mark.parametrize("g", [0.0, 20.0])
+def test_energy_loss_gradient_against_oracle_fd(oracle, g):
     """dE/dV of the final state's energy (tsmi.losses.energy_torch) through the
     autograd bridge -- the reverse sweep plus the direct potential term --
     against central differences of the oracle's evolution and energy at 12
-    seeded cells (h = 1e-4; closed axes, g = 0).  Bar: 1e-6 of the largest |FD|."""
+    seeded cells (h = 1e-4; closed axes; g = 0 and g = 20, the energy then
+    with its interaction term).  Bar: 1e-6 of the largest |FD|."""
     import torch
     from tsmi import autograd as TA
     from tsmi.hip import HipKernel
     from tsmi.losses import energy_torch
-    wl, _ = case(56, 48, 40)
+    wl, _ = case(56, 48, 40, g=g)
     lat = wl.lattice
     dev = torch.device("cuda:0")
-    k = HipKernel(lat, delta_t=wl.dt, precision=64)
+    k = HipKernel(lat, delta_t=wl.dt, precision=64, coupling=g)
     try:
         v = torch.tensor(wl.v, device=dev, requires_grad=True)
         p_re = torch.tensor(np.ascontiguousarray(wl.psi.real), device=dev)
         p_im = torch.tensor(np.ascontiguousarray(wl.psi.imag), device=dev)
         re, im = TA.evolve(k, v, p_re, p_im, wl.steps, every=8)
-        loss = energy_torch(lat, re, im, v, wl.mass)
+        loss = energy_torch(lat, re, im, v, wl.mass, g=g)
         loss.backward()
         torch.cuda.synchronize()
         grad = v.grad.cpu().numpy()

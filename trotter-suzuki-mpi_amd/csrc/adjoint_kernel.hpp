@@ -1,26 +1,32 @@
 // adjoint_kernel.hpp -- one reverse step of the real-time evolution, with the
 // gradient of the loss with respect to the potential.
 //
-// No reference counterpart: the reference evolves forward only.  For the linear
-// real-time step (closed axes, no nonlinearity) psi_{n+1} = U psi_n with
-// U = A P A', A' the first four kinetic passes V0 H0 V1 H1, P = diag(E),
-// E = exp(-i dt V), and A the last four.  Every factor is unitary, so
-//     U^H = A'^H P^H A^H
-// is the same nine-pass sequence with b -> -b in every pair and conj(E) in P:
-// the step at -dt.  With the adjoint state G = dL/dRe psi + i dL/dIm psi,
-//     G_n = U^H G_{n+1},   dL/dV_j += dt Im(conj(mu_j) psit_j),
-// where mu = A^H G_{n+1} and psit = A^H psi_{n+1} (= E chi_n, chi_n = A' psi_n)
-// are the two states at the middle of the reverse step.  The kernel therefore
-// runs the step at -dt on two states at once -- G, and psi_{n+1}, which comes
-// back as psi_n -- and at the potential stage adds dt Im(conj(mu) psit) into an
-// FP64 gradient plane.  The host re-anchors psi on the checkpoints of the
-// forward sweep, so reversal round-off never accumulates over more than one
-// checkpoint interval.
+// No reference counterpart: the reference evolves forward only.  A real-time
+// step is psi_{n+1} = A N(A' psi_n), A' the first four kinetic passes
+// V0 H0 V1 H1, A the last four, N the potential stage
+// N(chi) = E chi e^(-i phi(|chi|^2)), phi(s) = c s + c_LHY s^(3/2),
+// E = exp(-i dt V).  Every kinetic pass is unitary and symmetric, so its inverse
+// and its adjoint are the pass with b -> -b; |N(chi)| = |chi|, so N^-1(y) =
+// conj(E) e^(i phi(|y|^2)) y.  The step at -dt (b -> -b, conj(E), -phi) is
+// therefore the inverse step.  With the adjoint state
+// G = dL/dRe psi + i dL/dIm psi and the two states at the middle of the reverse
+// step, mu = A^H G_{n+1} and y = A^-1 psi_{n+1} (the forward's post-potential
+// state), at every cell
+//     dL/dV += dt Im(conj(mu) y),
+//     N^T mu = conj(E) e^(i phi) mu + 2 phi'(s) Im(conj(mu) y) x,
+// x = N^-1(y), s = |y|^2, phi'(s) = c + 1.5 c_LHY s^(1/2); with c = c_LHY = 0
+// this is conj(E) mu.  The kernel runs the step at -dt on two states at once --
+// G, and psi_{n+1}, which comes back as psi_n -- and at the potential stage
+// adds dt Im(conj(mu) y) into an FP64 gradient plane.  The host re-anchors psi
+// on the checkpoints of the forward sweep, so reversal round-off never
+// accumulates over more than one checkpoint interval.
 //
 // The layout, strip/segment decomposition and pipeline are those of
 // step_kernel; the scaled-frame factors (m, t, un) are the host's values for
-// the step at -dt, and gscale = dt |kappa|^2 undoes the deferred per-pass
-// factors of the two mid-step states (kappa = (gH gV)^2, |kappa|^2 = |corr|).
+// the step at -dt.  In that frame the two mid-step states are y/conj(kappa)
+// and mu/conj(kappa), kappa = (gH gV)^2, so gscale = dt |kappa|^2 restores the
+// gradient, the couplings are the forward's mid-step ones (c |kappa|^2,
+// c_LHY |kappa|^3) and the E planes carry kappa^2.
 #pragma once
 #include "step_kernel.hpp"
 
@@ -37,6 +43,7 @@ struct AdjParams {
     int W, H, pitch;
     int n_strips, n_waves, seg_rows;
     T mH, tH, mV, tV, unH, unV;      // frame factors of the step at -dt
+    T c_nl, c_lhy;                   // couplings in the mid-step frame
     double gscale;
 };
 
@@ -52,6 +59,31 @@ __device__ __forceinline__ void conj_e_mul(Row<T, VEC> &r, const Row<T, VEC> &e)
         const T pr = r.re[k], pi = r.im[k];
         r.re[k] = e.re[k] * pr + e.im[k] * pi;
         r.im[k] = e.re[k] * pi - e.im[k] * pr;
+    }
+}
+
+// N^-1 on the state row s (y -> x) and N^T on the adjoint row g (mu), in place.
+template <typename T, int VEC>
+__device__ __forceinline__ void nl_reverse(Row<T, VEC> &s, Row<T, VEC> &g, const Row<T, VEC> &e, T c, T cl) {
+    T ang[VEC], f[VEC], sn[VEC], cs[VEC];
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        const T n2 = s.re[k] * s.re[k] + s.im[k] * s.im[k];
+        const T r = sqrt_t(n2);
+        ang[k] = c * n2 + cl * (n2 * r);
+        f[k] = T(2) * (c + T(1.5) * cl * r) * (g.re[k] * s.im[k] - g.im[k] * s.re[k]);
+    }
+    sincos_row<T, VEC>(ang, sn, cs);
+    conj_e_mul(s, e);
+    conj_e_mul(g, e);
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        const T sr = cs[k] * s.re[k] - sn[k] * s.im[k], si = sn[k] * s.re[k] + cs[k] * s.im[k];
+        const T gr = cs[k] * g.re[k] - sn[k] * g.im[k], gi = sn[k] * g.re[k] + cs[k] * g.im[k];
+        s.re[k] = sr;
+        s.im[k] = si;
+        g.re[k] = fma(f[k], sr, gr);
+        g.im[k] = fma(f[k], si, gi);
     }
 }
 
@@ -80,7 +112,7 @@ __device__ __forceinline__ void buf_store_f64(__amdgpu_buffer_rsrc_t r, unsigned
     }
 }
 
-template <typename T, int VEC>
+template <typename T, int VEC, bool NL>
 __global__ void __launch_bounds__(256) adjoint_kernel(AdjParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
@@ -182,8 +214,12 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjParams<T> p) {
                             gr[k]);
             buf_store_f64<T, VEC>(r_grad, v_gd, (unsigned)(y2 - y_lo) * row_bytes_d, gr);
         }
-        conj_e_mul(o2.s, e);
-        conj_e_mul(o2.g, e);
+        if constexpr (NL) {
+            nl_reverse<T, VEC>(o2.s, o2.g, e, p.c_nl, p.c_lhy);
+        } else {
+            conj_e_mul(o2.s, e);
+            conj_e_mul(o2.g, e);
+        }
         hpass<(TP + 1) & 1, false>(o2.s, hc);
         hpass<(TP + 1) & 1, false>(o2.g, hc);
         // stage 3: V1, then H0 on row t-3

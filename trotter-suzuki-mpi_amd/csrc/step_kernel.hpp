@@ -967,4 +967,52 @@ __global__ void __launch_bounds__(256) halo_exchange_kernel(T *re, T *im, int W,
         exchange_cell<T>(i, re, im, W, H, pitch, px, py, hx, hy);
 }
 
+// The transpose of the exchange on a field of derivatives (G_src += G_dst,
+// G_dst = 0 for every copy, the exchange's two stages in reverse order): stage
+// 0 folds the y-halo rows (full width), stage 1 the x-halo columns of the rows
+// the exchange refreshes.  Each source cell has one destination per stage when
+// the inner extent is at least twice the halo on a periodic axis.
+__host__ __device__ __forceinline__ int fold_cells(int W, int H, int px, int py, int hx, int hy, int stage) {
+    if (stage == 0) return py ? 2 * hy * W : 0;
+    return px ? 2 * hx * max(0, H - 2 * hy) : 0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) halo_fold_kernel(T *re, T *im, int W, int H, int pitch, int px, int py,
+                                                        int hx, int hy, int stage) {
+    const int n = fold_cells(W, H, px, py, hx, hy, stage);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        long d, s;
+        if (stage == 0) {
+            const int r = i / W, x = i - r * W;
+            const int dy = r < hy ? r : H - 2 * hy + r;
+            const int sy = r < hy ? dy + (H - 2 * hy) : dy - (H - 2 * hy);
+            d = (long)dy * pitch + x;
+            s = (long)sy * pitch + x;
+        } else {
+            const int r = i / (2 * hx), c = i - r * 2 * hx;
+            const int y = (py ? hy : 0) + r;
+            const int dx = c < hx ? c : W - 2 * hx + c;
+            const int sx = c < hx ? dx + (W - 2 * hx) : dx - (W - 2 * hx);
+            d = (long)y * pitch + dx;
+            s = (long)y * pitch + sx;
+        }
+        re[s] += re[d];
+        im[s] += im[d];
+        re[d] = T(0);
+        im[d] = T(0);
+    }
+}
+
+// Zeros the cells of a W x H field outside the inner region [x0, x1) x [y0, y1).
+template <typename T>
+__global__ void __launch_bounds__(256) zero_outside_kernel(T *p, int W, int H, int pitch, int x0, int x1, int y0,
+                                                           int y1) {
+    const long n = (long)W * H;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int y = (int)(i / W), x = (int)(i - (long)y * W);
+        if (x < x0 || x >= x1 || y < y0 || y >= y1) p[(long)y * pitch + x] = T(0);
+    }
+}
+
 }  // namespace tsmi

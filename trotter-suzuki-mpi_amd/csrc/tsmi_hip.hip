@@ -598,7 +598,7 @@ int wrap_exp(tsmi_ctx *c) {
 }
 
 // One reverse step: (s_in, g_in) -> (s_out, g_out), gradient into c->grad.
-template <typename T, int VEC>
+template <typename T, int VEC, bool NL>
 int launch_adjoint(tsmi_ctx *c, const void *s_re, const void *s_im, const void *g_re, const void *g_im,
                    void *so_re, void *so_im, void *go_re, void *go_im) {
     AdjParams<T> p;
@@ -613,25 +613,82 @@ int launch_adjoint(tsmi_ctx *c, const void *s_re, const void *s_im, const void *
     p.n_strips = c->n_strips; p.n_waves = c->n_waves; p.seg_rows = c->seg_rows;
     p.mH = (T)c->amH; p.tH = (T)c->atH; p.mV = (T)c->amV; p.tV = (T)c->atV;
     p.unH = (T)c->aunH; p.unV = (T)c->aunV;
+    p.c_nl = (T)c->c_nl_s; p.c_lhy = (T)c->c_lhy_s;
     p.gscale = c->d.delta_t * c->corr_abs;
     const int blocks = (c->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
-    hipLaunchKernelGGL((adjoint_kernel<T, VEC>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
+    hipLaunchKernelGGL((adjoint_kernel<T, VEC, NL>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
     TSMI_HIP(hipGetLastError());
     return TSMI_OK;
 }
 
 int adjoint_step(tsmi_ctx *c, const void *s_re, const void *s_im, const void *g_re, const void *g_im,
                  void *so_re, void *so_im, void *go_re, void *go_im) {
-    return c->esz == 4 ? launch_adjoint<float, 4>(c, s_re, s_im, g_re, g_im, so_re, so_im, go_re, go_im)
-                       : launch_adjoint<double, 2>(c, s_re, s_im, g_re, g_im, so_re, so_im, go_re, go_im);
+    const bool nl = c->c_nl != 0.0 || c->c_lhy != 0.0;
+    if (c->esz == 4)
+        return nl ? launch_adjoint<float, 4, true>(c, s_re, s_im, g_re, g_im, so_re, so_im, go_re, go_im)
+                  : launch_adjoint<float, 4, false>(c, s_re, s_im, g_re, g_im, so_re, so_im, go_re, go_im);
+    return nl ? launch_adjoint<double, 2, true>(c, s_re, s_im, g_re, g_im, so_re, so_im, go_re, go_im)
+              : launch_adjoint<double, 2, false>(c, s_re, s_im, g_re, g_im, so_re, so_im, go_re, go_im);
+}
+
+// The halo exchange, its transpose, and the clearing of the halo, on a W x H
+// field of the reference tile's layout (origin at `re`, row pitch c->pitch):
+// the reverse sweep's view of a periodic lattice, whatever the allocation.
+template <typename T>
+int view_exchange(tsmi_ctx *c, void *re, void *im) {
+    const long cells = exchange_cells(c->W, c->H, c->d.periodic_x, c->d.periodic_y, c->d.halo_x, c->d.halo_y);
+    if (cells == 0) return TSMI_OK;
+    const int eb = (int)std::min<long>(1024, (cells + 255) / 256);
+    hipLaunchKernelGGL((halo_exchange_kernel<T>), dim3(eb), dim3(256), 0, c->stream, (T *)re, (T *)im, c->W, c->H,
+                       c->pitch, c->d.periodic_x, c->d.periodic_y, c->d.halo_x, c->d.halo_y);
+    TSMI_HIP(hipGetLastError());
+    return TSMI_OK;
+}
+
+template <typename T>
+int view_fold(tsmi_ctx *c, void *re, void *im) {
+    for (int stage = 0; stage < 2; ++stage) {
+        const long cells = fold_cells(c->W, c->H, c->d.periodic_x, c->d.periodic_y, c->d.halo_x, c->d.halo_y, stage);
+        if (cells == 0) continue;
+        const int eb = (int)std::min<long>(1024, (cells + 255) / 256);
+        hipLaunchKernelGGL((halo_fold_kernel<T>), dim3(eb), dim3(256), 0, c->stream, (T *)re, (T *)im, c->W, c->H,
+                           c->pitch, c->d.periodic_x, c->d.periodic_y, c->d.halo_x, c->d.halo_y, stage);
+        TSMI_HIP(hipGetLastError());
+    }
+    return TSMI_OK;
+}
+
+template <typename T>
+int view_zero_halo(tsmi_ctx *c, void *p) {
+    const long n = (long)c->W * c->H;
+    const int eb = (int)std::min<long>(2048, (n + 255) / 256);
+    hipLaunchKernelGGL((zero_outside_kernel<T>), dim3(eb), dim3(256), 0, c->stream, (T *)p, c->W, c->H, c->pitch,
+                       c->ix0, c->ix1, c->iy0, c->iy1);
+    TSMI_HIP(hipGetLastError());
+    return TSMI_OK;
+}
+
+int view_exchange_any(tsmi_ctx *c, void *re, void *im) {
+    return c->esz == 4 ? view_exchange<float>(c, re, im) : view_exchange<double>(c, re, im);
 }
 
 // The reverse sweep over the steps of the last checkpointed evolve.  g_re/g_im
 // are FP64 W x H planes on the device; the gradient is left in c->grad and the
 // adjoint of the initial state in the returned planes.
+//
+// Periodic axes: the tile with a refreshed halo evolves as the periodic lattice
+// (the halo is as wide as one step's reach), so the sweep runs on the periodic
+// lattice: dL/dpsi_N's halo entries are folded onto the cells they mirror and
+// the halo then holds their copies, psi_N gets its halo refreshed (the forward's
+// last step leaves it stale), and after every reverse step both states get
+// their halo refreshed.  The gradient and dL/dpsi_0 are those of the inner
+// cells -- each with respect to the value a cell shares with its halo copies --
+// and zero on the halo.
 int reverse_sweep(tsmi_ctx *c, const double *g_re, const double *g_im, void **g0_re, void **g0_im) {
     const size_t pb = plane_bytes(c);
     const size_t pd = (size_t)c->pitch * (size_t)c->Hd * sizeof(double);
+    const size_t view_off = c->off_el * (size_t)c->esz;
+    const bool per = c->d.periodic_x || c->d.periodic_y;
     if (!c->adj) TSMI_HIP(hipMalloc(&c->adj, 8 * pb));
     if (!c->grad) TSMI_HIP(hipMalloc((void **)&c->grad, pd));
     char *a = (char *)c->adj;
@@ -644,20 +701,41 @@ int reverse_sweep(tsmi_ctx *c, const double *g_re, const double *g_im, void **g0
     if (!st) st = import_device(c, L[0][1], g_im);
     if (st) return st;
     const void *s_re = c->psi_re[c->cur], *s_im = c->psi_im[c->cur];
+    if (per) {
+        st = c->esz == 4 ? view_fold<float>(c, L[0][0], L[0][1]) : view_fold<double>(c, L[0][0], L[0][1]);
+        if (!st) st = view_exchange_any(c, L[0][0], L[0][1]);
+        const size_t row = (size_t)c->pitch * c->esz;
+        TSMI_HIP(hipMemcpy2DAsync(S[0][0], row, s_re, row, (size_t)c->W * c->esz, c->H, hipMemcpyDeviceToDevice,
+                                  c->stream));
+        TSMI_HIP(hipMemcpy2DAsync(S[0][1], row, s_im, row, (size_t)c->W * c->esz, c->H, hipMemcpyDeviceToDevice,
+                                  c->stream));
+        if (!st) st = view_exchange_any(c, S[0][0], S[0][1]);
+        if (st) return st;
+        s_re = S[0][0];
+        s_im = S[0][1];
+    }
     int k = 0;
     for (int n = c->ck_steps - 1; n >= 0; --n) {
         const int o = 1 - k;
         st = adjoint_step(c, s_re, s_im, L[k][0], L[k][1], S[o][0], S[o][1], L[o][0], L[o][1]);
+        if (!st && per) st = view_exchange_any(c, L[o][0], L[o][1]);
         if (st) return st;
         if (n % c->ck_every == 0) {   // psi_n is a checkpoint: use it instead of the reversed state
-            const char *ck = (const char *)c->ck + (size_t)(n / c->ck_every) * 2 * pb;
+            const char *ck = (const char *)c->ck + (size_t)(n / c->ck_every) * 2 * pb + view_off;
             s_re = ck;
             s_im = ck + pb;
         } else {
+            if (per && (st = view_exchange_any(c, S[o][0], S[o][1]))) return st;
             s_re = S[o][0];
             s_im = S[o][1];
         }
         k = o;
+    }
+    if (per) {
+        st = view_zero_halo<double>(c, c->grad);
+        if (!st) st = c->esz == 4 ? view_zero_halo<float>(c, L[k][0]) : view_zero_halo<double>(c, L[k][0]);
+        if (!st) st = c->esz == 4 ? view_zero_halo<float>(c, L[k][1]) : view_zero_halo<double>(c, L[k][1]);
+        if (st) return st;
     }
     *g0_re = L[k][0];
     *g0_im = L[k][1];
@@ -829,7 +907,12 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     c->mode = k.mode;
     c->corr_re = k.corr_re; c->corr_im = k.corr_im;
     c->c_nl_s = k.c_nl_s; c->c_lhy_s = k.c_lhy_s;
-    c->adj_ok = !d.imag_time && !d.periodic_x && !d.periodic_y && d.coupling == 0.0 && d.coupling_lhy == 0.0;
+    // the reverse sweep: real time; on a periodic axis an inner extent of at
+    // least twice the halo, and no x-periodic tile whose exchange leaves rows
+    // unrefreshed (closed y with a y halo: the reference refreshes H - 2 halo_y
+    // rows), which would not evolve as a periodic lattice
+    c->adj_ok = !d.imag_time && (!d.periodic_x || d.dim_x >= 2 * d.halo_x) &&
+                (!d.periodic_y || d.dim_y >= 2 * d.halo_y) && !(d.periodic_x && !d.periodic_y && d.halo_y > 0);
     c->amH = k.amH; c->atH = k.atH; c->aunH = k.aunH;
     c->amV = k.amV; c->atV = k.atV; c->aunV = k.aunV;
     c->corr_abs = k.corr_abs;
@@ -1039,7 +1122,8 @@ int tsmi_evolve_checkpointed(tsmi_ctx *c, int32_t n, int32_t every) {
         return fail(TSMI_E_INVALID, "tsmi_evolve_checkpointed: state and exp potential must be set");
     if (!c->adj_ok)
         return fail(TSMI_E_UNSUPPORTED,
-                    "tsmi_evolve_checkpointed: the reverse sweep needs real time, closed axes and zero couplings");
+                    "tsmi_evolve_checkpointed: the reverse sweep needs real time and, on periodic axes, an inner "
+                    "extent of at least twice the halo and no x-periodic tile with a closed y axis");
     const size_t pb = plane_bytes(c);
     const int count = n > 0 ? (n - 1) / every + 1 : 0;
     const size_t need = 2 * (size_t)count * pb;
@@ -1061,7 +1145,8 @@ int tsmi_evolve_checkpointed(tsmi_ctx *c, int32_t n, int32_t every) {
             TSMI_HIP(hipMemcpyAsync(dst + pb, c->psi_base[c->cur][1], pb, hipMemcpyDeviceToDevice, c->stream));
         }
         const bool two = pairs && i + 1 < n && (i + 1) % every != 0;
-        const int st = two ? step2_once(c) : step_once(c, 0);
+        // periodic: the halo exchange after every step but the last, as evolve
+        const int st = two ? step2_once(c) : step_once(c, i + 1 < n ? 1 : 0);
         if (st) return st;
         i += two ? 2 : 1;
     }

@@ -6,9 +6,11 @@ steps on the device with E = exp(-i dt V) built from the FP64 tensor ``v`` and
 returns the final state as two FP64 tensors.  Its backward pass is the
 library's reverse sweep (tsmi_adjoint_device): the step at -dt applied to the
 adjoint and to the state together, re-anchored on checkpoints kept every
-``every`` steps, with dL/dV accumulated in FP64.  Real time, closed axes and
-g = g_LHY = 0 only; the kernel's precision (32 or 64) is the precision of both
-sweeps.  Everything runs on torch's current stream.
+``every`` steps, with dL/dV accumulated in FP64.  Real time, any g and g_LHY,
+closed or periodic axes as tsmi_adjoint allows (on periodic axes the gradients
+are those of the periodic fields: each inner cell's value shared with its halo
+copies, zero on the halo); the kernel's precision (32 or 64) is the precision
+of both sweeps.  Everything runs on torch's current stream.
 """
 from __future__ import annotations
 
