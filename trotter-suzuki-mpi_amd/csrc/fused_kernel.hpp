@@ -358,7 +358,8 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
 template <typename T, int VEC, bool LHY, int MODE, bool PH = false>
 __global__ void __launch_bounds__(256, (sizeof(T) == 4 && PH && MODE >= 0) ? TSMI_STEP2_OCC : 1) step2_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
+    const int wave = __builtin_amdgcn_readfirstlane(
+        xcd_block<sizeof(T) == 4>(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
     if constexpr (TSMI_STREAM && sizeof(T) == 4 && VEC == 2 && PH && MODE >= 0) {
         constexpr int AP = 2 * kApron;

@@ -396,7 +396,8 @@ __device__ __forceinline__ void pair_wave(const PairParams<T> &p, const int wave
 template <typename T, bool IMAG, int VEC, bool PH>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) pair_kernel(PairParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
+    const int wave = __builtin_amdgcn_readfirstlane(xcd_block<true>(blockIdx.x, gridDim.x) * (blockDim.x / kWave) +
+                                                    (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
     pair_wave<T, IMAG, VEC, PH>(p, wave, lane);
 }
@@ -596,7 +597,8 @@ __device__ __forceinline__ void pair2_wave(const PairParams<T> &p, const int wav
 template <typename T, int VEC, bool PH>
 __global__ void __launch_bounds__(256, 2) pair2_kernel(PairParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave));
+    const int wave = __builtin_amdgcn_readfirstlane(xcd_block<true>(blockIdx.x, gridDim.x) * (blockDim.x / kWave) +
+                                                    (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
     pair2_wave<T, VEC, PH>(p, wave, lane);
 }
