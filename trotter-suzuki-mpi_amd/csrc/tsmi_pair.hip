@@ -142,7 +142,16 @@ int launch_pair_step(tsmi_pair *p, double var, bool rotate) {
     }
     constexpr int VEC = sizeof(T) == 4 ? kVecPair32 : 2;
     const int blocks = (p->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
-    hipLaunchKernelGGL((pair_kernel<T, IMAG, VEC, PH>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
+    // FP64 real time: the fast beat needs one scaled form for both components on both axes
+    const int mode = p->c[0]->mode == p->c[1]->mode ? p->c[0]->mode : -1;
+    if constexpr (IMAG || sizeof(T) == 4)
+        hipLaunchKernelGGL((pair_kernel<T, IMAG, VEC, PH, -1>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
+    else if (mode == 0)
+        hipLaunchKernelGGL((pair_kernel<T, IMAG, VEC, PH, 0>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
+    else if (mode == 1)
+        hipLaunchKernelGGL((pair_kernel<T, IMAG, VEC, PH, 1>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
+    else
+        hipLaunchKernelGGL((pair_kernel<T, IMAG, VEC, PH, -1>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
     TSMI_HIPP(hipGetLastError());
     p->c[0]->cur = nb;
     p->c[1]->cur = nb;
@@ -258,7 +267,14 @@ int pair_step2(tsmi_pair *p, double var1, double var2, bool rotate) {
         q.rabi = 1;
     }
     const int blocks = (p->n_waves2 + kBlock / kWave - 1) / (kBlock / kWave);
-    hipLaunchKernelGGL((pair2_kernel<T, kVecPair32, PH>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
+    // the fast beat needs one scaled form for both components on both axes
+    const int mode = p->c[0]->mode == p->c[1]->mode ? p->c[0]->mode : -1;
+    if (mode == 0)
+        hipLaunchKernelGGL((pair2_kernel<T, kVecPair32, PH, 0>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
+    else if (mode == 1)
+        hipLaunchKernelGGL((pair2_kernel<T, kVecPair32, PH, 1>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
+    else
+        hipLaunchKernelGGL((pair2_kernel<T, kVecPair32, PH, -1>), dim3(blocks), dim3(kBlock), 0, p->c[0]->stream, q);
     TSMI_HIPP(hipGetLastError());
     p->c[0]->cur = nb;
     p->c[1]->cur = nb;
@@ -308,7 +324,8 @@ int check_ready(tsmi_pair *p, const char *who) {
 template <typename T, bool IMAG>
 hipError_t pair_occupancy(int *blocks) {
     constexpr int VEC = sizeof(T) == 4 ? kVecPair32 : 2;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, pair_kernel<T, IMAG, VEC, true>, kBlock, 0);
+    constexpr int MODE = (IMAG || sizeof(T) == 4) ? -1 : 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, pair_kernel<T, IMAG, VEC, true, MODE>, kBlock, 0);
 }
 
 hipError_t occupancy_of(int esz, bool imag, int *blocks) {
@@ -405,7 +422,7 @@ int tsmi_pair_create(const tsmi_pair_desc *pd, tsmi_pair **out) {
             int dev = 0, cus = 0, per_cu = 0;
             if (hipGetDevice(&dev) == hipSuccess &&
                 hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pair2_kernel<float, kVecPair32, true>, kBlock,
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pair2_kernel<float, kVecPair32, true, 0>, kBlock,
                                                              0) == hipSuccess &&
                 cus > 0 && per_cu > 0) {
                 const long resident = (long)cus * per_cu * (kBlock / kWave);
