@@ -355,12 +355,10 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
 #ifndef TSMI_STEP2_OCC
 #define TSMI_STEP2_OCC 4
 #endif
-template <typename T, int VEC, bool LHY, int MODE, bool PH = false>
-__global__ void __launch_bounds__(256, (sizeof(T) == 4 && PH && MODE >= 0) ? TSMI_STEP2_OCC : 1) step2_kernel(StepParams<T> p) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(
-        xcd_block<sizeof(T) == 4>(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave));
-    if (wave >= p.n_waves) return;
+// One wave's share of a two-step pass: the FP32 stream where it applies (an
+// interior segment of an interior strip), step2_wave otherwise.
+template <typename T, int VEC, bool LHY, int MODE, bool PH>
+__device__ __forceinline__ void step2_item(const StepParams<T> &p, const int wave, const int lane) {
     if constexpr (TSMI_STREAM && sizeof(T) == 4 && VEC == 2 && PH && MODE >= 0) {
         constexpr int AP = 2 * kApron;
         constexpr int SW = kWave * VEC - 2 * AP;
@@ -388,6 +386,19 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 4 && PH && MODE >= 0) ? TSM
         }
     }
     step2_wave<T, VEC, LHY, MODE, PH>(p, wave, lane);
+}
+
+// Waves per SIMD the FP32 phase-plane two-step instances are bounded to (128
+// VGPRs: the stream's register slots without spills).
+constexpr int step2_occupancy(int esz, bool ph, int mode) { return (esz == 4 && ph && mode >= 0) ? TSMI_STEP2_OCC : 1; }
+
+template <typename T, int VEC, bool LHY, int MODE, bool PH = false>
+__global__ void __launch_bounds__(256, step2_occupancy(sizeof(T), PH, MODE)) step2_kernel(StepParams<T> p) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(
+        xcd_block<sizeof(T) == 4>(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave));
+    if (wave >= p.n_waves) return;
+    step2_item<T, VEC, LHY, MODE, PH>(p, wave, lane);
 }
 
 }  // namespace tsmi
