@@ -37,9 +37,10 @@ struct PairParams {
     T mH[2], tH[2], mV[2], tV[2], unH[2], unV[2];   // scaled-frame pair factors per component
     T c_self[2];                     // dt g of each component, in its mid-step frame
     T c_ab;                          // dt g_ab (the other density is a stored, true-frame value)
-    // Real-time Rabi rotation of the rows leaving the pipelines, in FP64 at every
-    // precision: R = [[c, i s], [i conj(s), c]] has R^H R = (c^2 + |s|^2) I, so
-    // constants rounded to FP32 would scale the norm by the same factor every step.
+    // Real-time Rabi rotation of the rows leaving the pipelines (constants in
+    // FP64; R = [[c, i s], [i conj(s), c]] has R^H R = (c^2 + |s|^2) I, so an FP32
+    // c rounded near 1 would scale the norm by the same factor every step -- the
+    // FP32 kernels use c = 1 - d, see rabi_rows).
     double cc, cs_r, cs_i;
     double cc2, cs_r2, cs_i2;        // the second step's rotation in a two-step pass (pair2_kernel)
     int rabi;                        // 1: apply it
@@ -162,16 +163,33 @@ __device__ __forceinline__ void pair_pipe_fast(Pipe4<T, VEC> &q, Row<T, VEC> &in
 }
 
 // The real-time Rabi rotation of a pair of rows (src/cpucartesian.cpp:264-276,
-// every right-hand side the old value), in FP64.
+// every right-hand side the old value).  FP64 in the FP64 kernels.  FP32 with
+// c = 1 - d: each output is x - d x plus the two s terms, three fused
+// multiply-adds.  R^H R = ((1 - d)^2 + |s|^2) I then differs from I only by the
+// roundings of d and s relative to themselves -- about 1e-7 of |s|^2 ~ 2d, far
+// below one FP32 ulp per step -- where an FP32 c rounded near 1 would put a
+// factor off by up to 6e-8 on the norm every step.
 template <typename T, int VEC>
 __device__ __forceinline__ void rabi_rows(Row<T, VEC> &oa, Row<T, VEC> &ob, double cc, double sr, double si) {
+    if constexpr (sizeof(T) == 4) {
+        const float d = float(1.0 - cc), fr = float(sr), fi = float(si);
 #pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-        const double ar = oa.re[k], ai = oa.im[k], br = ob.re[k], bi = ob.im[k];
-        oa.re[k] = T(cc * ar - si * br - sr * bi);
-        oa.im[k] = T(cc * ai + sr * br - si * bi);
-        ob.re[k] = T(cc * br + si * ar - sr * ai);
-        ob.im[k] = T(cc * bi + sr * ar + si * ai);
+        for (int k = 0; k < VEC; ++k) {
+            const float ar = oa.re[k], ai = oa.im[k], br = ob.re[k], bi = ob.im[k];
+            oa.re[k] = fmaf(-d, ar, fmaf(-fi, br, fmaf(-fr, bi, ar)));
+            oa.im[k] = fmaf(-d, ai, fmaf(fr, br, fmaf(-fi, bi, ai)));
+            ob.re[k] = fmaf(-d, br, fmaf(fi, ar, fmaf(-fr, ai, br)));
+            ob.im[k] = fmaf(-d, bi, fmaf(fr, ar, fmaf(fi, ai, bi)));
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const double ar = oa.re[k], ai = oa.im[k], br = ob.re[k], bi = ob.im[k];
+            oa.re[k] = T(cc * ar - si * br - sr * bi);
+            oa.im[k] = T(cc * ai + sr * br - si * bi);
+            ob.re[k] = T(cc * br + si * ar - sr * ai);
+            ob.im[k] = T(cc * bi + sr * ar + si * ai);
+        }
     }
 }
 
