@@ -70,15 +70,12 @@ __device__ __forceinline__ void ppass_pair(Row<T, VEC> &r, const Row<T, VEC> &e,
     if (!IMAG) {
         T sn[VEC], cs[VEC];
         sincos_row<T, VEC>(ang, sn, cs);
+        if (PH) {
+            rotate_row_ph<T, VEC>(r, cs, sn, mag);
+        } else {
 #pragma unroll
-        for (int k = 0; k < VEC; ++k) {
-            const T pr = r.re[k], pi = r.im[k];
-            if (PH) {
-                T c2, s2;
-                phase_factor<T>(cs[k], sn[k], mag, c2, s2);
-                r.re[k] = c2 * pr + s2 * pi;
-                r.im[k] = c2 * pi - s2 * pr;
-            } else {
+            for (int k = 0; k < VEC; ++k) {
+                const T pr = r.re[k], pi = r.im[k];
                 const T r1 = e.re[k] * pr - e.im[k] * pi;
                 const T i1 = e.re[k] * pi + e.im[k] * pr;
                 r.re[k] = cs[k] * r1 + sn[k] * i1;

@@ -444,29 +444,40 @@ __device__ __forceinline__ void expneg_row(const T (&x)[VEC], T (&e)[VEC]) {
     }
 }
 
-// mag (cos, sin) of the phase form, rounded to the kernel precision.  In FP32
-// the factor is k = mag (1 + e/2), e = 1 - cos^2 - sin^2 (first-order
-// renormalisation of the rounded pair), and each output is rounded once from
-// cos * k: when mag is within an ulp of 1, cos * mag rounded alone would drop
-// it at almost every site (a uniform norm bias of ~6e-9 per step), while the
-// per-site e dithers the rounding.  mag = m1 + m2 in two floats, so no
-// rounding of mag is applied uniformly either: cos * k = fma(cos, m1, cos * u)
-// with u = m2 + m1 e / 2, a term ~1e-7 of the result whose own FP32 rounding
-// stays below 1e-14.  1 - cos^2 - sin^2 comes from two fused multiply-adds
-// (each square exact inside its fma).
-template <typename T>
-__device__ __forceinline__ void phase_factor(T cs, T sn, double mag, T &c2, T &s2) {
-    if constexpr (sizeof(T) == 4) {
-        const float m1 = (float)mag;
-        const float m2 = (float)(mag - (double)m1);
-        float e = fmaf(-cs, cs, 1.0f);
-        e = fmaf(-sn, sn, e);
-        const float u = fmaf(0.5f * m1, e, m2);
-        c2 = fmaf(cs, m1, cs * u);
-        s2 = fmaf(sn, m1, sn * u);
-    } else {
-        c2 = cs * mag;
-        s2 = sn * mag;
+// The phase-form rotation of a row, p -> mag e^(-i phi) p, from the rounded
+// (cos, sin) of phi.  FP64: the factors cos*mag and sin*mag, then the rotation.
+// FP32: k = mag (1 + e/2), e = 1 - cos^2 - sin^2, renormalises the rounded
+// pair to first order (e from two fused multiply-adds, each square exact inside
+// its fma), and mag = m1 + m2 is held in two floats so that no rounding of mag
+// is applied uniformly.  k multiplies the state -- p k = fma(m1, p, u p),
+// u = m2 + m1 e / 2, one rounding of a product that is inexact at almost every
+// site -- and the raw (cos, sin) rotate it.  Forming cos*k and sin*k instead
+// rounds them near the same values at every site of a slowly varying phase
+// (where |phi| is small, cos rounds to 1 and cos*k to m1 everywhere): a bias of
+// up to 1e-8 per step on the norm, against at most 1e-10 for p k (CPU
+// emulation of both forms on random states and angles, |mag - 1| > 1e-6; within
+// 1e-6 of 1, where k p rounds back to p, both leave up to 6e-9 at tiny angles).
+template <typename T, int VEC>
+__device__ __forceinline__ void rotate_row_ph(Row<T, VEC> &r, const T (&cs)[VEC], const T (&sn)[VEC],
+                                              double mag) {
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        if constexpr (sizeof(T) == 4) {
+            const float m1 = (float)mag;
+            const float m2 = (float)(mag - (double)m1);
+            float e = fmaf(-cs[k], cs[k], 1.0f);
+            e = fmaf(-sn[k], sn[k], e);
+            const float u = fmaf(0.5f * m1, e, m2);
+            const float qr = fmaf(m1, r.re[k], u * r.re[k]);
+            const float qi = fmaf(m1, r.im[k], u * r.im[k]);
+            r.re[k] = fmaf(cs[k], qr, sn[k] * qi);
+            r.im[k] = fmaf(cs[k], qi, -(sn[k] * qr));
+        } else {
+            const T c2 = cs[k] * mag, s2 = sn[k] * mag;
+            const T pr = r.re[k], pi = r.im[k];
+            r.re[k] = c2 * pr + s2 * pi;
+            r.im[k] = c2 * pi - s2 * pr;
+        }
     }
 }
 
@@ -487,15 +498,12 @@ __device__ __forceinline__ void ppass(Row<T, VEC> &r, const Row<T, VEC> &e, T c,
     if (!IMAG) {
         T sn[VEC], cs[VEC];
         sincos_row<T, VEC>(ang, sn, cs);
+        if (PH) {
+            rotate_row_ph<T, VEC>(r, cs, sn, mag);
+        } else {
 #pragma unroll
-        for (int k = 0; k < VEC; ++k) {
-            const T pr = r.re[k], pi = r.im[k];
-            if (PH) {
-                T c2, s2;
-                phase_factor<T>(cs[k], sn[k], mag, c2, s2);
-                r.re[k] = c2 * pr + s2 * pi;
-                r.im[k] = c2 * pi - s2 * pr;
-            } else {
+            for (int k = 0; k < VEC; ++k) {
+                const T pr = r.re[k], pi = r.im[k];
                 const T r1 = e.re[k] * pr - e.im[k] * pi;
                 const T i1 = e.re[k] * pi + e.im[k] * pr;
                 r.re[k] = cs[k] * r1 + sn[k] * i1;
