@@ -169,3 +169,31 @@ def test_rabi_transfer_closed_form(precision, tol):
     th = np.hypot(1.2, -0.9) / 2 * pw.dt * pw.steps
     assert e[9] == pytest.approx(n0 * np.cos(th) ** 2, rel=tol, abs=tol)
     assert e[10] == pytest.approx(n0 * np.sin(th) ** 2, rel=tol, abs=tol)
+
+
+@pytest.mark.parametrize("sweeps", [1, 2])
+def test_pair_fp32_norm_drift_with_rabi(sweeps):
+    """FP32 total norm over 400 steps of two overlapping interacting clouds
+    with Rabi coupling (512^2 periodic, g_a = g_b = 50, g_ab = 40, omega = 1,
+    dt = 5e-4), one and two steps per pass: within 1e-7 of the initial norm,
+    relative.  The rotation's cosine 1 - d (d = 3e-8) differs from 1 by less
+    than half an FP32 ulp of the state: -d x applied in a rounding of its own
+    is lost at most sites while the sine terms still add |s|^2, a drift of 2d
+    per step (2.4e-5 over these steps)."""
+    from tsmi.lattice import Lattice2D
+    from tsmi.pair import HipPair
+    lat = Lattice2D(512, 8.0, periodic_x=True, periodic_y=True)
+    X, Y = lat.mesh()
+    v_a = 0.5 * 0.64 * (X * X + Y * Y)
+    v_b = 0.5 * 0.768 * (X * X + Y * Y)
+    amp = np.sqrt(np.maximum(10.0 - v_a, 0.0) / 50.0)
+    psi_a = (amp * np.exp(1j * 2.4 * X)).astype(np.complex128)
+    psi_b = (0.5 * amp * np.exp(-1j * 1.6 * Y)).astype(np.complex128)
+    pw = W.PairWorkload("pair_drift_512", lat, psi_a, psi_b, v_a, v_b, 5e-4, (400,), g_a=50.0, g_b=50.0,
+                        g_ab=40.0, omega_r=1.0)
+    n0 = W.squared_norm(lat, psi_a) + W.squared_norm(lat, psi_b)
+    with HipPair(pw, precision=32, sweeps=sweeps) as h:
+        h.evolve(400)
+        e = h.energy()
+    drift = e[11] / n0 - 1
+    assert abs(drift) < 1e-7, drift

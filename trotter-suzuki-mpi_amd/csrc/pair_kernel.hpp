@@ -161,11 +161,13 @@ __device__ __forceinline__ void pair_pipe_fast(Pipe4<T, VEC> &q, Row<T, VEC> &in
 
 // The real-time Rabi rotation of a pair of rows (src/cpucartesian.cpp:264-276,
 // every right-hand side the old value).  FP64 in the FP64 kernels.  FP32 with
-// c = 1 - d: each output is x - d x plus the two s terms, three fused
-// multiply-adds.  R^H R = ((1 - d)^2 + |s|^2) I then differs from I only by the
-// roundings of d and s relative to themselves -- about 1e-7 of |s|^2 ~ 2d, far
-// below one FP32 ulp per step -- where an FP32 c rounded near 1 would put a
-// factor off by up to 6e-8 on the norm every step.
+// c = 1 - d: each output is x plus the increment -d x + (the two s terms),
+// formed in FP32 first and added to x in one rounding.  R^H R = ((1 - d)^2 +
+// |s|^2) I then differs from I only by the roundings of d and s relative to
+// themselves (about 1e-7 of |s|^2 ~ 2d).  An FP32 c rounded near 1 would put a
+// factor off by up to 6e-8 on the norm every step, and so would -d x applied in
+// a rounding of its own: d (3e-8 at Omega dt = 5e-4) is below half an ulp of x,
+// so x - d x rounds back to x at most sites while the s terms still add |s|^2.
 template <typename T, int VEC>
 __device__ __forceinline__ void rabi_rows(Row<T, VEC> &oa, Row<T, VEC> &ob, double cc, double sr, double si) {
     if constexpr (sizeof(T) == 4) {
@@ -173,10 +175,14 @@ __device__ __forceinline__ void rabi_rows(Row<T, VEC> &oa, Row<T, VEC> &ob, doub
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
             const float ar = oa.re[k], ai = oa.im[k], br = ob.re[k], bi = ob.im[k];
-            oa.re[k] = fmaf(-d, ar, fmaf(-fi, br, fmaf(-fr, bi, ar)));
-            oa.im[k] = fmaf(-d, ai, fmaf(fr, br, fmaf(-fi, bi, ai)));
-            ob.re[k] = fmaf(-d, br, fmaf(fi, ar, fmaf(-fr, ai, br)));
-            ob.im[k] = fmaf(-d, bi, fmaf(fr, ar, fmaf(fi, ai, bi)));
+            const float ua = fmaf(-d, ar, fmaf(-fi, br, -(fr * bi)));
+            const float va = fmaf(-d, ai, fmaf(fr, br, -(fi * bi)));
+            const float ub = fmaf(-d, br, fmaf(fi, ar, -(fr * ai)));
+            const float vb = fmaf(-d, bi, fmaf(fr, ar, fi * ai));
+            oa.re[k] = ar + ua;
+            oa.im[k] = ai + va;
+            ob.re[k] = br + ub;
+            ob.im[k] = bi + vb;
         }
     } else {
 #pragma unroll
