@@ -110,6 +110,18 @@ __device__ __forceinline__ void unroll_beats(F &&f, std::integer_sequence<int, K
 #define TSMI_STREAM 1
 #endif
 constexpr int kStreamAhead = TSMI_STREAM_AHEAD;
+// Beats between workgroup barriers in the stream (0: none; a multiple of the
+// period D + 8 beyond one period).  The four waves of a workgroup run four
+// neighbouring strips, which read the same apron columns of the same rows: kept
+// in step, they fetch those lines through L2 once (8192^2 FP32, per launch:
+// FETCH_SIZE 553581 -> 530514 KiB, 317.2 -> 308.6 us; a barrier every 1, 2, 3,
+// 4, 6, 24, 36, 48 beats measured 314.1, 310.9, 310.2, 309.7, 310.5, 309.4,
+// 310.3, 311.6 us; 8- and 16-wave workgroups 309.6 and 311.9 us).
+#ifndef TSMI_STREAM_SYNC
+#define TSMI_STREAM_SYNC 12
+#endif
+constexpr int kStreamSync = TSMI_STREAM_SYNC;
+constexpr int kStep2Block = 256;     // threads per step2_kernel workgroup (its launch bound)
 
 template <int TP, bool LHY, int MODE>
 __device__ __forceinline__ void stream_pipe(Row<float, 2> &r0, Row<float, 2> &r1, Row<float, 2> &r2,
@@ -132,7 +144,7 @@ __device__ __forceinline__ void step2_stream_f32(__amdgpu_buffer_rsrc_t r_in_re,
                                                  __amdgpu_buffer_rsrc_t r_ph, __amdgpu_buffer_rsrc_t r_out_re,
                                                  __amdgpu_buffer_rsrc_t r_out_im, unsigned v_ld, unsigned v_st,
                                                  unsigned row_bytes, int rows, int out_lo, int out_hi, float kH,
-                                                 float kV, float c, float cl, double mag) {
+                                                 float kV, float c, float cl, double mag, bool sync) {
     constexpr int D = kStreamAhead;
     constexpr int S = D + 8;
     static_assert(D >= 2 && S % 2 == 0, "pair parities need an even period");
@@ -174,6 +186,12 @@ __device__ __forceinline__ void step2_stream_f32(__amdgpu_buffer_rsrc_t r_in_re,
                 buf_store<float, 2>(r_out_im, off, 0u, out.im);
                 load_state(j + D, out);
                 load_phase(j + D - 2, e[(k + D - 2) % S]);
+                if constexpr (kStreamSync > 0 && kStreamSync < S && k % kStreamSync == kStreamSync - 1) {
+                    if (sync) __builtin_amdgcn_s_barrier();
+                } else if constexpr (kStreamSync >= S && k == S - 1) {
+                    static_assert(kStreamSync % S == 0, "a barrier interval beyond one period is whole periods");
+                    if (sync && (j0 / S) % (kStreamSync / S) == kStreamSync / S - 1) __builtin_amdgcn_s_barrier();
+                }
             },
             std::make_integer_sequence<int, S>());
     }
@@ -358,10 +376,18 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
 // One wave's share of a two-step pass: the FP32 stream where it applies (an
 // interior segment of an interior strip), step2_wave otherwise.
 template <typename T, int VEC, bool LHY, int MODE, bool PH>
-__device__ __forceinline__ void step2_item(const StepParams<T> &p, const int wave, const int lane) {
+__device__ __forceinline__ void step2_item(const StepParams<T> &p, const int wave, const int lane,
+                                           const int block_wave0 = -1) {
     if constexpr (TSMI_STREAM && sizeof(T) == 4 && VEC == 2 && PH && MODE >= 0) {
         constexpr int AP = 2 * kApron;
         constexpr int SW = kWave * VEC - 2 * AP;
+        // interior strip and segment: the stream form
+        auto stream_ok = [&](int w) {
+            const int st = w % p.n_strips, sg = w / p.n_strips;
+            const int x0 = st * SW - AP, lo = sg * p.seg_rows;
+            return w < p.n_waves && x0 >= 0 && x0 + kWave * VEC <= p.W && lo - AP > 0 &&
+                   min(p.H, lo + p.seg_rows) + AP < p.H;
+        };
         const int strip = wave % p.n_strips;
         const int seg = wave / p.n_strips;
         const int X0 = strip * SW - AP;
@@ -369,7 +395,16 @@ __device__ __forceinline__ void step2_item(const StepParams<T> &p, const int wav
         const int y_out_hi = min(p.H, y_out_lo + p.seg_rows);
         const int y_lo = y_out_lo - AP;
         const int y_hi = y_out_hi + AP;
-        if (X0 >= 0 && X0 + kWave * VEC <= p.W && y_lo > 0 && y_hi < p.H) {
+        if (stream_ok(wave)) {
+            // the workgroup's waves (neighbouring strips) keep step with a
+            // barrier every kStreamSync beats when all of them run the stream
+            // (the same segment height, so the same number of barriers)
+            bool sync = false;
+            if constexpr (kStreamSync > 0) {
+                sync = block_wave0 >= 0;
+#pragma unroll
+                for (int i = 0; i < kStep2Block / kWave; ++i) sync = sync && stream_ok(block_wave0 + i);
+            }
             const int xl = X0 + lane * VEC;
             const bool own = lane >= AP / VEC && lane < kWave - AP / VEC;
             const long seg0 = (long)y_lo * p.pitch;
@@ -381,7 +416,7 @@ __device__ __forceinline__ void step2_item(const StepParams<T> &p, const int wav
                                         make_rsrc(p.ph + seg0, seg_bytes), make_rsrc(p.out_re + seg0, seg_bytes),
                                         make_rsrc(p.out_im + seg0, seg_bytes), v_ld, v_st,
                                         (unsigned)p.pitch * (unsigned)sizeof(T), y_hi - y_lo, AP,
-                                        AP + (y_out_hi - y_out_lo), kH, kV, p.c_nl, p.c_lhy, p.mag);
+                                        AP + (y_out_hi - y_out_lo), kH, kV, p.c_nl, p.c_lhy, p.mag, sync);
             return;
         }
     }
@@ -393,12 +428,12 @@ __device__ __forceinline__ void step2_item(const StepParams<T> &p, const int wav
 constexpr int step2_occupancy(int esz, bool ph, int mode) { return (esz == 4 && ph && mode >= 0) ? TSMI_STEP2_OCC : 1; }
 
 template <typename T, int VEC, bool LHY, int MODE, bool PH = false>
-__global__ void __launch_bounds__(256, step2_occupancy(sizeof(T), PH, MODE)) step2_kernel(StepParams<T> p) {
+__global__ void __launch_bounds__(kStep2Block, step2_occupancy(sizeof(T), PH, MODE)) step2_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(
-        xcd_block<sizeof(T) == 4>(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave));
+    const int wave0 = xcd_block<sizeof(T) == 4>(blockIdx.x, gridDim.x) * (blockDim.x / kWave);
+    const int wave = __builtin_amdgcn_readfirstlane(wave0 + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
-    step2_item<T, VEC, LHY, MODE, PH>(p, wave, lane);
+    step2_item<T, VEC, LHY, MODE, PH>(p, wave, lane, wave0);
 }
 
 }  // namespace tsmi
