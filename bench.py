@@ -73,6 +73,8 @@ def parse():
                     help="only run that study and print its JSON object (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (for counter calibration) and exit")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="only time plain elementwise kernels over planes of the 8192^2 FP32 tile and exit")
     return ap.parse_args()
 
 
@@ -85,6 +87,36 @@ def calibrate_copy():
         torch.mul(a, 1.0, out=b)
     torch.cuda.synchronize()
     print(json.dumps({"copy_kernel_bytes_read": 4 * n, "copy_kernel_bytes_written": 4 * n, "launches": 5}))
+
+
+def stream_ceiling(launches: int = 20):
+    """What plain streaming kernels reach on this GPU over planes the size of the
+    8192^2 FP32 tile with its 8-wide halo (8256 x 8208 floats): a copy (one
+    plane read, one written), an add (two read, one written) and addcmul (three
+    read, one written), each timed with HIP events over `launches` launches."""
+    import torch
+    n = 8256 * 8208
+    planes = [torch.rand(n, device="cuda", dtype=torch.float32) for _ in range(3)]
+    out = torch.empty_like(planes[0])
+    a, b, c = planes
+    kernels = {"copy": (lambda: torch.mul(a, 1.0, out=out), 2),
+               "add": (lambda: torch.add(a, b, out=out), 3),
+               "addcmul": (lambda: torch.addcmul(c, a, b, out=out), 4)}
+    res = {"plane_bytes": 4 * n, "launches": launches}
+    for name, (fn, nplanes) in kernels.items():
+        for _ in range(3):
+            fn()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0.record()
+        for _ in range(launches):
+            fn()
+        t1.record()
+        torch.cuda.synchronize()
+        us = t0.elapsed_time(t1) * 1e3 / launches
+        res[name] = {"us": us, "bytes": nplanes * 4 * n, "TB_per_s": nplanes * 4 * n / us / 1e6}
+    print(json.dumps({"stream_ceiling": res}), flush=True)
 
 
 def time_precision(prec, lat, psi_re, psi_im, e_re, e_im, v, params, steps, warmup, tile_rows):
@@ -459,12 +491,15 @@ def main():
               "(multi-GPU sharding is a design note, not part of this build); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
-    if args.calibrate_copy or args.only_study:
+    if args.calibrate_copy or args.stream_ceiling or args.only_study:
         if not torch.cuda.is_available():
             print("bench.py needs a GPU", file=sys.stderr)
             sys.exit(2)
         if args.calibrate_copy:
             calibrate_copy()
+            return
+        if args.stream_ceiling:
+            stream_ceiling()
             return
         fn = {"precision_study": precision_study, "batch": batch_study, "adjoint": adjoint_study,
               "pair": pair_study, "observables": observables_study}[args.only_study]
