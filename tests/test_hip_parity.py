@@ -225,6 +225,22 @@ def test_two_steps_per_pass_fp32(oracle):
     assert rel_l2(a, b) < 1e-5
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_two_steps_per_pass_short_segments_periodic(precision, oracle):
+    """Doubly periodic, two steps per pass with 12-row segments: interior
+    segments (the FP32 stream) then have output rows within 8 rows of the edge,
+    and the 8-wide halo they read is refreshed between passes.  Against the
+    oracle (FP64 1e-12, FP32 2e-5) and single steps (FP32 1e-6)."""
+    wl = W.small_case(480, 120, True, True, steps=9, g=2.0)
+    ref = oracle.evolve(wl)
+    a, *_ = run_hip(wl, oracle, precision=precision, sweeps=2, tile_rows=12)
+    b, *_ = run_hip(wl, oracle, precision=precision, sweeps=1)
+    tol = 1e-12 if precision == 64 else 2e-5
+    assert rel_l2(a, ref) < tol and rel_l2(b, ref) < tol
+    if precision == 32:
+        assert rel_l2(a, b) < 1e-6
+
+
 def test_two_steps_per_pass_rejected_where_not_exact():
     from tsmi import _lib
     from tsmi.hip import kernel_for

@@ -504,8 +504,19 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) pair_kernel(PairP
 // rows twice (mostly from the caches) and writes psi_a, psi_b once.  Exactness
 // across tiles follows the single-component pass: closed axes, or periodic axes
 // with the 8-wide halo refreshed before each pass and E wrapped over it.
+#ifndef TSMI_PAIR2_SYNC
+#define TSMI_PAIR2_SYNC 4
+#endif
+// Beats between workgroup barriers in the fast loop of pair2_wave (0: none;
+// even): the workgroup's four waves run neighbouring strips, which read the same
+// apron columns, and in step they fetch those lines through L2 once.
+constexpr int kPair2Sync = TSMI_PAIR2_SYNC;
+static_assert(kPair2Sync % 2 == 0, "the fast loop runs two beats per iteration");
+constexpr int kPair2Block = 256;     // threads per pair2_kernel workgroup (its launch bound)
+
 template <typename T, int VEC, bool PH, int MODE>
-__device__ __forceinline__ void pair2_wave(const PairParams<T> &p, const int wave, const int lane) {
+__device__ __forceinline__ void pair2_wave(const PairParams<T> &p, const int wave, const int lane,
+                                           const int block_wave0 = -1) {
     constexpr int AP = 2 * kApron;
     constexpr int SW = kWave * VEC - 2 * AP;
     const int strip = wave % p.n_strips;
@@ -642,8 +653,22 @@ __device__ __forceinline__ void pair2_wave(const PairParams<T> &p, const int wav
     // apron rows only, and a wrong value moves at most one row per vertical
     // pass, eight per pass of two steps, so it never reaches an output row.
     if constexpr (MODE >= 0) {
-        const int X0s = strip * SW - AP;
-        if (X0s >= 0 && X0s + kWave * VEC <= p.W && y_out_lo >= AP && y_out_hi + AP <= p.H) {
+        // strip inside the tile, apron rows inside the lattice: the fast loop
+        auto fast_ok = [&](int w) {
+            const int st = w % p.n_strips, sg = w / p.n_strips;
+            const int x0 = st * SW - AP, lo = sg * p.seg_rows;
+            return w < p.n_waves && x0 >= 0 && x0 + kWave * VEC <= p.W && lo >= AP &&
+                   min(p.H, lo + p.seg_rows) + AP <= p.H;
+        };
+        if (fast_ok(wave)) {
+            // all four waves of the workgroup in the fast loop (the same segment
+            // height, so the same trip count): a barrier every kPair2Sync beats
+            bool sync = false;
+            if constexpr (kPair2Sync > 0) {
+                sync = block_wave0 >= 0;
+#pragma unroll
+                for (int i = 0; i < kPair2Block / kWave; ++i) sync = sync && fast_ok(block_wave0 + i);
+            }
             const T kH0 = MODE == 0 ? p.tH[0] : p.mH[0], kV0 = MODE == 0 ? p.tV[0] : p.mV[0];
             const T kH1 = MODE == 0 ? p.tH[1] : p.mH[1], kV1 = MODE == 0 ? p.tV[1] : p.mV[1];
             // E rows by row index; a row outside the segment reads zeros
@@ -687,7 +712,7 @@ __device__ __forceinline__ void pair2_wave(const PairParams<T> &p, const int wav
             int t = y_lo;
             load_psi(t, ia0, ib0);
             if (t + 1 < y_hi) load_psi(t + 1, ia1, ib1);
-            for (; t < y_hi + 8; t += 2) {
+            for (int it = 0; t < y_hi + 8; t += 2, ++it) {
                 beat_fast(I0(), t, ia0, ib0, ea0, eb0, fa0, fb0, na0, nb0, ma0, mb0);
                 load_psi_any(t + 2, ia0, ib0);
                 load_e_at(t, ea0, eb0);
@@ -696,6 +721,9 @@ __device__ __forceinline__ void pair2_wave(const PairParams<T> &p, const int wav
                 load_psi_any(t + 3, ia1, ib1);
                 load_e_at(t + 1, ea1, eb1);
                 load_e_at(t - 3, fa1, fb1);
+                if constexpr (kPair2Sync > 0) {
+                    if (sync && it % (kPair2Sync / 2) == kPair2Sync / 2 - 1) __builtin_amdgcn_s_barrier();
+                }
             }
             return;
         }
@@ -748,12 +776,12 @@ __device__ __forceinline__ void pair2_wave(const PairParams<T> &p, const int wav
 // MODE: the scaled form both components use on both axes (0 or 1), or -1 when
 // they differ (the general beat only).
 template <typename T, int VEC, bool PH, int MODE>
-__global__ void __launch_bounds__(256, 2) pair2_kernel(PairParams<T> p) {
+__global__ void __launch_bounds__(kPair2Block, 2) pair2_kernel(PairParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(xcd_block<true>(blockIdx.x, gridDim.x) * (blockDim.x / kWave) +
-                                                    (threadIdx.x / kWave));
+    const int wave0 = xcd_block<true>(blockIdx.x, gridDim.x) * (blockDim.x / kWave);
+    const int wave = __builtin_amdgcn_readfirstlane(wave0 + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
-    pair2_wave<T, VEC, PH, MODE>(p, wave, lane);
+    pair2_wave<T, VEC, PH, MODE>(p, wave, lane, wave0);
 }
 
 // ---------------------------------------------------------------------------

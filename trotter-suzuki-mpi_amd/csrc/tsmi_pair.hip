@@ -29,6 +29,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kVecPair32 = 2;   // FP32 columns per lane of pair_kernel and pair2_kernel (FP64: 2)
+static_assert(kBlock == kPair2Block, "pair2_kernel's workgroup size is its launch bound");
 constexpr int kRabiBlocksMax = 1024;
 
 #define TSMI_HIPP(call)                                                                              \
