@@ -191,6 +191,7 @@ FUSED = [
     dict(n_x=601, n_y=75, periodic_x=False, periodic_y=False, g=1.0),            # ragged, closed
     dict(n_x=8, n_y=8, periodic_x=True, periodic_y=True),
     dict(n_x=13, n_y=7, periodic_x=False, periodic_y=False, g=0.5),
+    dict(n_x=45, n_y=37, periodic_x=True, periodic_y=True, g=1.0),               # odd: the per-cell exchange
 ]
 
 
