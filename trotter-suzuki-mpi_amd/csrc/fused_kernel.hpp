@@ -122,6 +122,12 @@ constexpr int kStreamAhead = TSMI_STREAM_AHEAD;
 #endif
 constexpr int kStreamSync = TSMI_STREAM_SYNC;
 constexpr int kStep2Block = 256;     // threads per step2_kernel workgroup (its launch bound)
+// Cache policy of the passes' output rows: non-temporal in FP64 (8192^2, per
+// launch: 604.4 -> 588.0 us, three interleaved rounds; also 608.3 -> 586.5 on
+// another GPU); the default in FP32, where the FP32 pass measured 308.3 -> 313.0
+// us with it on one GPU and 308.6 -> 305.4 us on another.
+template <typename T>
+constexpr int step2_store() { return sizeof(T) == 8 ? kStoreNT : 0; }
 
 template <int TP, bool LHY, int MODE>
 __device__ __forceinline__ void stream_pipe(Row<float, 2> &r0, Row<float, 2> &r1, Row<float, 2> &r2,
@@ -182,8 +188,8 @@ __device__ __forceinline__ void step2_stream_f32(__amdgpu_buffer_rsrc_t r_in_re,
                 Row<float, 2> &out = s[(k + S - 8) % S];
                 const int y = j - 8;
                 const unsigned off = (y >= out_lo && y < out_hi) ? v_st + row_off(y) : kOffNone;
-                buf_store<float, 2>(r_out_re, off, 0u, out.re);
-                buf_store<float, 2>(r_out_im, off, 0u, out.im);
+                buf_store<float, 2, step2_store<float>()>(r_out_re, off, 0u, out.re);
+                buf_store<float, 2, step2_store<float>()>(r_out_im, off, 0u, out.im);
                 load_state(j + D, out);
                 load_phase(j + D - 2, e[(k + D - 2) % S]);
                 if constexpr (kStreamSync > 0 && kStreamSync < S && k % kStreamSync == kStreamSync - 1) {
@@ -261,8 +267,8 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
     }
     auto emit = [&](int y, const Row<T, VEC> &o) {
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
-        buf_store<T, VEC>(r_out_re, v_st, so, o.re);
-        buf_store<T, VEC>(r_out_im, v_st, so, o.im);
+        buf_store<T, VEC, step2_store<T>()>(r_out_re, v_st, so, o.re);
+        buf_store<T, VEC, step2_store<T>()>(r_out_im, v_st, so, o.im);
     };
 
     // beat t: pipeline A takes input row t (E row t-2) and gives row t-4, the
@@ -304,8 +310,8 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
     };
     auto emit_any = [&](int y, const Row<T, VEC> &o) {
         const unsigned off = y >= y_out_lo ? v_st + (unsigned)(y - y_lo) * row_bytes : kOffNone;
-        buf_store<T, VEC>(r_out_re, off, 0u, o.re);
-        buf_store<T, VEC>(r_out_im, off, 0u, o.im);
+        buf_store<T, VEC, step2_store<T>()>(r_out_re, off, 0u, o.re);
+        buf_store<T, VEC, step2_store<T>()>(r_out_im, off, 0u, o.im);
     };
     auto pair_fast = [&](int t) {
         Row<T, VEC> a, b;

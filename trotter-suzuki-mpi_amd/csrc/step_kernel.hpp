@@ -590,14 +590,23 @@ __device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff
 // 128-bit store then carries a one-wait-state hazard on its data registers
 // that the compiler models (it does not for a store with a scalar-register
 // offset).
-template <typename T, int VEC>
+// Store cache policy: kStoreNT marks the row non-temporal (streamed out of L2
+// first), which leaves L2 to the input rows that neighbouring waves share.
+constexpr int kStoreNT = 2;
+// step_wave's output rows: non-temporal in FP64 real time (8192^2 per launch:
+// 599.0 -> 581.5 us); the default elsewhere (FP32 real time 307.2 -> 304.8 us
+// with it on one GPU, but the FP32 two-step pass is slower with it on others;
+// imaginary time at 2048^2 26.5 -> 29.1 us FP32, 43.8 -> 47.6 us FP64).
+template <typename T, bool IMAG>
+constexpr int step_store() { return (!IMAG && sizeof(T) == 8) ? kStoreNT : 0; }
+template <typename T, int VEC, int CPOL = 0>
 __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const T (&v)[VEC]) {
     if constexpr (sizeof(T) * VEC == 8) {
         typedef T __attribute__((ext_vector_type(VEC))) TV;
         TV x;
 #pragma unroll
         for (int k = 0; k < VEC; ++k) x[k] = v[k];
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, voff + soff, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, voff + soff, 0, CPOL);
     } else {
         constexpr int P = sizeof(T) * VEC / 16;
         constexpr int E = 16 / sizeof(T);
@@ -607,7 +616,7 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
             TV x;
 #pragma unroll
             for (int k = 0; k < E; ++k) x[k] = v[i * E + k];
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, voff + soff + 16 * i, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, voff + soff + 16 * i, 0, CPOL);
         }
     }
 }
@@ -730,8 +739,8 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     };
     auto emit = [&](int y, const Row<T, VEC> &o4) {
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
-        buf_store<T, VEC>(r_out_re, v_st, so, o4.re);
-        buf_store<T, VEC>(r_out_im, v_st, so, o4.im);
+        buf_store<T, VEC, step_store<T, IMAG>()>(r_out_re, v_st, so, o4.re);
+        buf_store<T, VEC, step_store<T, IMAG>()>(r_out_im, v_st, so, o4.im);
         accumulate(y, o4);
     };
     // The emit of the fast form: a row outside [y_out_lo, y_out_hi) is dropped
@@ -739,8 +748,8 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     auto emit_any = [&](int y, const Row<T, VEC> &o4) {
         const bool out = y >= y_out_lo && y < y_out_hi;
         const unsigned off = out ? v_st + (unsigned)(y - y_lo) * row_bytes : kOffNone;
-        buf_store<T, VEC>(r_out_re, off, 0u, o4.re);
-        buf_store<T, VEC>(r_out_im, off, 0u, o4.im);
+        buf_store<T, VEC, step_store<T, IMAG>()>(r_out_re, off, 0u, o4.re);
+        buf_store<T, VEC, step_store<T, IMAG>()>(r_out_im, off, 0u, o4.im);
         if (out) accumulate(y, o4);
     };
 
