@@ -34,6 +34,7 @@ struct tsmi_ctx {
     int Wd = 0, Hd = 0;
     size_t off_el = 0;                    // elements from allocation to view origin
     bool halo8 = false;
+    bool nt_out = false;                  // FP64 real time: output rows stored non-temporal
     // kernel timing (tsmi_kernel_timing): one event pair per step-kernel launch
     bool timing = false;
     std::vector<hipEvent_t> ev[2][2];     // [fused?][start/end]

@@ -122,12 +122,6 @@ constexpr int kStreamAhead = TSMI_STREAM_AHEAD;
 #endif
 constexpr int kStreamSync = TSMI_STREAM_SYNC;
 constexpr int kStep2Block = 256;     // threads per step2_kernel workgroup (its launch bound)
-// Cache policy of the passes' output rows: non-temporal in FP64 (8192^2, per
-// launch: 604.4 -> 588.0 us, three interleaved rounds; also 608.3 -> 586.5 on
-// another GPU); the default in FP32, where the FP32 pass measured 308.3 -> 313.0
-// us with it on one GPU and 308.6 -> 305.4 us on another.
-template <typename T>
-constexpr int step2_store() { return sizeof(T) == 8 ? kStoreNT : 0; }
 
 template <int TP, bool LHY, int MODE>
 __device__ __forceinline__ void stream_pipe(Row<float, 2> &r0, Row<float, 2> &r1, Row<float, 2> &r2,
@@ -188,8 +182,8 @@ __device__ __forceinline__ void step2_stream_f32(__amdgpu_buffer_rsrc_t r_in_re,
                 Row<float, 2> &out = s[(k + S - 8) % S];
                 const int y = j - 8;
                 const unsigned off = (y >= out_lo && y < out_hi) ? v_st + row_off(y) : kOffNone;
-                buf_store<float, 2, step2_store<float>()>(r_out_re, off, 0u, out.re);
-                buf_store<float, 2, step2_store<float>()>(r_out_im, off, 0u, out.im);
+                buf_store<float, 2>(r_out_re, off, 0u, out.re);
+                buf_store<float, 2>(r_out_im, off, 0u, out.im);
                 load_state(j + D, out);
                 load_phase(j + D - 2, e[(k + D - 2) % S]);
                 if constexpr (kStreamSync > 0 && kStreamSync < S && k % kStreamSync == kStreamSync - 1) {
@@ -206,7 +200,7 @@ __device__ __forceinline__ void step2_stream_f32(__amdgpu_buffer_rsrc_t r_in_re,
 // Two real-time steps of one wavefront's strip segment.  p.seg_rows, p.n_strips
 // and p.n_waves describe the fused decomposition (strips of kWave*VEC - 16
 // output columns).
-template <typename T, int VEC, bool LHY, int MODE, bool PH = false>
+template <typename T, int VEC, bool LHY, int MODE, bool PH = false, int SC = 0>
 __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wave, const int lane) {
     constexpr int AP = 2 * kApron;
     constexpr int SW = kWave * VEC - 2 * AP;
@@ -267,8 +261,8 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
     }
     auto emit = [&](int y, const Row<T, VEC> &o) {
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
-        buf_store<T, VEC, step2_store<T>()>(r_out_re, v_st, so, o.re);
-        buf_store<T, VEC, step2_store<T>()>(r_out_im, v_st, so, o.im);
+        buf_store<T, VEC, SC>(r_out_re, v_st, so, o.re);
+        buf_store<T, VEC, SC>(r_out_im, v_st, so, o.im);
     };
 
     // beat t: pipeline A takes input row t (E row t-2) and gives row t-4, the
@@ -310,8 +304,8 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
     };
     auto emit_any = [&](int y, const Row<T, VEC> &o) {
         const unsigned off = y >= y_out_lo ? v_st + (unsigned)(y - y_lo) * row_bytes : kOffNone;
-        buf_store<T, VEC, step2_store<T>()>(r_out_re, off, 0u, o.re);
-        buf_store<T, VEC, step2_store<T>()>(r_out_im, off, 0u, o.im);
+        buf_store<T, VEC, SC>(r_out_re, off, 0u, o.re);
+        buf_store<T, VEC, SC>(r_out_im, off, 0u, o.im);
     };
     auto pair_fast = [&](int t) {
         Row<T, VEC> a, b;
@@ -381,7 +375,7 @@ __device__ __forceinline__ void step2_wave(const StepParams<T> &p, const int wav
 #endif
 // One wave's share of a two-step pass: the FP32 stream where it applies (an
 // interior segment of an interior strip), step2_wave otherwise.
-template <typename T, int VEC, bool LHY, int MODE, bool PH>
+template <typename T, int VEC, bool LHY, int MODE, bool PH, int SC = 0>
 __device__ __forceinline__ void step2_item(const StepParams<T> &p, const int wave, const int lane,
                                            const int block_wave0 = -1) {
     if constexpr (TSMI_STREAM && sizeof(T) == 4 && VEC == 2 && PH && MODE >= 0) {
@@ -426,20 +420,21 @@ __device__ __forceinline__ void step2_item(const StepParams<T> &p, const int wav
             return;
         }
     }
-    step2_wave<T, VEC, LHY, MODE, PH>(p, wave, lane);
+    step2_wave<T, VEC, LHY, MODE, PH, SC>(p, wave, lane);
 }
 
 // Waves per SIMD the FP32 phase-plane two-step instances are bounded to (128
 // VGPRs: the stream's register slots without spills).
 constexpr int step2_occupancy(int esz, bool ph, int mode) { return (esz == 4 && ph && mode >= 0) ? TSMI_STEP2_OCC : 1; }
 
-template <typename T, int VEC, bool LHY, int MODE, bool PH = false>
+// NT: output rows stored non-temporal (see kStoreNT).
+template <typename T, int VEC, bool LHY, int MODE, bool PH = false, bool NT = false>
 __global__ void __launch_bounds__(kStep2Block, step2_occupancy(sizeof(T), PH, MODE)) step2_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave0 = xcd_block<sizeof(T) == 4>(blockIdx.x, gridDim.x) * (blockDim.x / kWave);
     const int wave = __builtin_amdgcn_readfirstlane(wave0 + (threadIdx.x / kWave));
     if (wave >= p.n_waves) return;
-    step2_item<T, VEC, LHY, MODE, PH>(p, wave, lane, wave0);
+    step2_item<T, VEC, LHY, MODE, PH, NT ? kStoreNT : 0>(p, wave, lane, wave0);
 }
 
 }  // namespace tsmi

@@ -593,12 +593,16 @@ __device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff
 // Store cache policy: kStoreNT marks the row non-temporal (streamed out of L2
 // first), which leaves L2 to the input rows that neighbouring waves share.
 constexpr int kStoreNT = 2;
-// step_wave's output rows: non-temporal in FP64 real time (8192^2 per launch:
-// 599.0 -> 581.5 us); the default elsewhere (FP32 real time 307.2 -> 304.8 us
-// with it on one GPU, but the FP32 two-step pass is slower with it on others;
-// imaginary time at 2048^2 26.5 -> 29.1 us FP32, 43.8 -> 47.6 us FP64).
-template <typename T, bool IMAG>
-constexpr int step_store() { return (!IMAG && sizeof(T) == 8) ? kStoreNT : 0; }
+// The host asks for it (template NT of step_kernel and step2_kernel) in FP64
+// real time when a step's rows (state in and out, phase plane) exceed the
+// 256 MB Infinity Cache: there the output is not read back from that cache, and
+// the policy measured FP64 per step at 4096^2 87.7 -> 85.7 us, 8192^2 309.7 ->
+// 302 us (two-step passes) and 595 -> 573 us (one-step); at 1024^2 and 2048^2,
+// where the next step finds the rows in that cache, it measured slower (15.9
+// -> 16.7 and 29.9 -> 31.0 us two-step, 18.2 -> 21.2 and 37.5 -> 44.5 us
+// one-step).  FP32 keeps the default: its two-step pass measured 308.3 -> 313.0
+// us with it at 8192^2 on two GPUs (305.4 on a third); imaginary time keeps it
+// too (2048^2: 26.5 -> 29.1 us FP32, 43.8 -> 47.6 us FP64).
 template <typename T, int VEC, int CPOL = 0>
 __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const T (&v)[VEC]) {
     if constexpr (sizeof(T) * VEC == 8) {
@@ -628,7 +632,7 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
 // (wave / n_strips).  `wave` is wave-uniform.
 // BLOCK_SUM (imaginary time): the wave returns its partial sum in *blk instead
 // of storing it, and the caller combines the partials of its workgroup.
-template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false, bool BLOCK_SUM = false>
+template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false, bool BLOCK_SUM = false, int SC = 0>
 __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave, const int lane,
                                           double *blk = nullptr) {
     constexpr int SW = kWave * VEC - 2 * kApron;       // output columns per strip
@@ -739,8 +743,8 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     };
     auto emit = [&](int y, const Row<T, VEC> &o4) {
         const unsigned so = (unsigned)(y - y_lo) * row_bytes;
-        buf_store<T, VEC, step_store<T, IMAG>()>(r_out_re, v_st, so, o4.re);
-        buf_store<T, VEC, step_store<T, IMAG>()>(r_out_im, v_st, so, o4.im);
+        buf_store<T, VEC, SC>(r_out_re, v_st, so, o4.re);
+        buf_store<T, VEC, SC>(r_out_im, v_st, so, o4.im);
         accumulate(y, o4);
     };
     // The emit of the fast form: a row outside [y_out_lo, y_out_hi) is dropped
@@ -748,8 +752,8 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     auto emit_any = [&](int y, const Row<T, VEC> &o4) {
         const bool out = y >= y_out_lo && y < y_out_hi;
         const unsigned off = out ? v_st + (unsigned)(y - y_lo) * row_bytes : kOffNone;
-        buf_store<T, VEC, step_store<T, IMAG>()>(r_out_re, off, 0u, o4.re);
-        buf_store<T, VEC, step_store<T, IMAG>()>(r_out_im, off, 0u, o4.im);
+        buf_store<T, VEC, SC>(r_out_re, off, 0u, o4.re);
+        buf_store<T, VEC, SC>(r_out_im, off, 0u, o4.im);
         if (out) accumulate(y, o4);
     };
 
@@ -922,7 +926,7 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
 // barrier at the end), so the next launch's waves each read a quarter as many
 // partials (the read is O(waves x partials) per step).  Every wave of the grid
 // reaches that barrier, those past the last segment included.
-template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false>
+template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false, bool NT = false>
 __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform by construction; readfirstlane lets the compiler keep every
@@ -931,7 +935,7 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
         xcd_block<IMAG || sizeof(T) == 8>(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave));
     if constexpr (!IMAG) {
         if (wave >= p.n_waves) return;
-        step_wave<T, IMAG, VEC, LHY, MODE, PH>(p, wave, lane);
+        step_wave<T, IMAG, VEC, LHY, MODE, PH, false, NT ? kStoreNT : 0>(p, wave, lane);
     } else {
         constexpr int kWavesPerBlock = 256 / kWave;
         __shared__ double part[kWavesPerBlock];

@@ -426,7 +426,11 @@ int launch_step2(tsmi_ctx *c) {
     p.c_nl = (T)c->c_nl_s; p.c_lhy = (T)c->c_lhy_s;
     const int blocks = (c->n_waves2 + kBlock / kWave - 1) / (kBlock / kWave);
     if (int st = time_mark(c, 1, 0)) return st;
-    hipLaunchKernelGGL((step2_kernel<T, VEC, LHY, MODE, PH>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
+    if (sizeof(T) == 8 && c->nt_out)
+        hipLaunchKernelGGL((step2_kernel<T, VEC, LHY, MODE, PH, sizeof(T) == 8>), dim3(blocks), dim3(kBlock), 0,
+                           c->stream, p);
+    else
+        hipLaunchKernelGGL((step2_kernel<T, VEC, LHY, MODE, PH>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
     TSMI_HIP(hipGetLastError());
     if (int st = time_mark(c, 1, 1)) return st;
     c->cur = nb;
@@ -482,7 +486,11 @@ int launch_step(tsmi_ctx *c, int exchange) {
     // the grid covers every (strip, segment) pair exactly once
     const int blocks = (c->n_waves + kBlock / kWave - 1) / (kBlock / kWave);
     if (int st = time_mark(c, 0, 0)) return st;
-    hipLaunchKernelGGL((step_kernel<T, IMAG, VEC, LHY, MODE, PH>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
+    if (!IMAG && sizeof(T) == 8 && c->nt_out)
+        hipLaunchKernelGGL((step_kernel<T, IMAG, VEC, LHY, MODE, PH, !IMAG && sizeof(T) == 8>), dim3(blocks),
+                           dim3(kBlock), 0, c->stream, p);
+    else
+        hipLaunchKernelGGL((step_kernel<T, IMAG, VEC, LHY, MODE, PH>), dim3(blocks), dim3(kBlock), 0, c->stream, p);
     TSMI_HIP(hipGetLastError());
     if (int st = time_mark(c, 0, 1)) return st;
     c->cur = nb;
@@ -901,6 +909,13 @@ int tsmi_create(const tsmi_desc *desc, tsmi_ctx **out) {
     c->Wd = (int)Wd; c->Hd = (int)Hd;
     c->halo8 = halo8;
     c->off_el = halo8 ? (size_t)d.halo_y * (size_t)pitch + (size_t)d.halo_x : 0;
+    // FP64 real time stores its output rows non-temporal when a step's rows
+    // (state in and out, phase plane: five planes) exceed the 256 MB Infinity
+    // Cache of the MI355X, which then cannot return them to the next step
+    {
+        const long long plane = pitch * Hd * (d.precision == 32 ? 4 : 8);
+        c->nt_out = d.precision == 64 && !d.imag_time && 5 * plane > (256LL << 20);
+    }
     c->sweeps = sweeps;
     c->esz = d.precision == 32 ? 4 : 8;
     c->vec = c->esz == 4 ? (d.imag_time ? kVec32I : kVec32) : 2;
