@@ -632,9 +632,22 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
 // (wave / n_strips).  `wave` is wave-uniform.
 // BLOCK_SUM (imaginary time): the wave returns its partial sum in *blk instead
 // of storing it, and the caller combines the partials of its workgroup.
+#ifndef TSMI_STEP_SYNC
+#define TSMI_STEP_SYNC 4
+#endif
+// Beats between workgroup barriers in step_wave's all-fast loop (FP32 real
+// time; 0: none; a multiple of its 4 beats per iteration): the workgroup's four
+// waves run neighbouring strips, which read the same apron columns, and in step
+// they fetch those lines through L2 once.  8192^2 FP32 per launch: 308.7 ->
+// 299.8 us (every 12 and 24 beats: 301.0, 302.3 us); FP64 564.4 -> 566.9 us with
+// it, so it keeps none.
+constexpr int kStepSync = TSMI_STEP_SYNC;
+static_assert(kStepSync % 4 == 0, "the all-fast loop runs four beats per iteration");
+constexpr int kStepBlock = 256;      // threads per step_kernel workgroup (its launch bound)
+
 template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false, bool BLOCK_SUM = false, int SC = 0>
 __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave, const int lane,
-                                          double *blk = nullptr) {
+                                          double *blk = nullptr, const int block_wave0 = -1) {
     constexpr int SW = kWave * VEC - 2 * kApron;       // output columns per strip
     const int strip = wave % p.n_strips;
     const int seg = wave / p.n_strips;
@@ -897,10 +910,29 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
     // moves it at most one row, four a step, so it never reaches an output row
     // (the 4-column strip aprons rest on the same bound).  The row offsets of
     // the last beats stay below 2^31.
-    const bool all_fast = interior && y_out_lo >= kApron && y_out_hi + kApron <= p.H &&
-                          (unsigned long)(y_hi - y_lo + 12) * row_bytes < 0x80000000ul;
+    const bool rows_ok = (unsigned long)(p.seg_rows + 2 * kApron + 12) * row_bytes < 0x80000000ul;
+    auto all_fast_of = [&](int w) {
+        const int st = w % p.n_strips, sg = w / p.n_strips;
+        const int x0 = st * SW - kApron, lo = sg * p.seg_rows;
+        return w < p.n_waves && MODE >= 0 && x0 >= 0 && x0 + kWave * VEC <= p.W && lo >= kApron &&
+               min(p.H, lo + p.seg_rows) + kApron <= p.H && rows_ok;
+    };
+    const bool all_fast = all_fast_of(wave);
     if (all_fast) {
-        for (; t < t_end; t += 4) group_fast(t);
+        // real time, all four waves of the workgroup here (the same segment
+        // height, so the same trip count): a barrier every kStepSync beats
+        bool sync = false;
+        if constexpr (!IMAG && sizeof(T) == 4 && kStepSync > 0) {
+            sync = block_wave0 >= 0;
+#pragma unroll
+            for (int i = 0; i < kStepBlock / kWave; ++i) sync = sync && all_fast_of(block_wave0 + i);
+        }
+        for (int it = 0; t < t_end; t += 4, ++it) {
+            group_fast(t);
+            if constexpr (!IMAG && sizeof(T) == 4 && kStepSync > 0) {
+                if (sync && it % (kStepSync / 4) == kStepSync / 4 - 1) __builtin_amdgcn_s_barrier();
+            }
+        }
     } else if (interior) {
         // rows y_lo .. y_lo+3: the pipeline fills
         for (; t < t_end && t < y_lo + 4; t += 4) group_general(t);
@@ -927,15 +959,15 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
 // partials (the read is O(waves x partials) per step).  Every wave of the grid
 // reaches that barrier, those past the last segment included.
 template <typename T, bool IMAG, int VEC, bool LHY, int MODE, bool PH = false, bool NT = false>
-__global__ void __launch_bounds__(256) step_kernel(StepParams<T> p) {
+__global__ void __launch_bounds__(kStepBlock) step_kernel(StepParams<T> p) {
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform by construction; readfirstlane lets the compiler keep every
     // quantity derived from it (segment, strip, row bounds) in scalar registers
-    const int wave = __builtin_amdgcn_readfirstlane(
-        xcd_block<IMAG || sizeof(T) == 8>(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + (threadIdx.x / kWave));
+    const int wave0 = xcd_block<IMAG || sizeof(T) == 8>(blockIdx.x, gridDim.x) * (blockDim.x / kWave);
+    const int wave = __builtin_amdgcn_readfirstlane(wave0 + (threadIdx.x / kWave));
     if constexpr (!IMAG) {
         if (wave >= p.n_waves) return;
-        step_wave<T, IMAG, VEC, LHY, MODE, PH, false, NT ? kStoreNT : 0>(p, wave, lane);
+        step_wave<T, IMAG, VEC, LHY, MODE, PH, false, NT ? kStoreNT : 0>(p, wave, lane, nullptr, wave0);
     } else {
         constexpr int kWavesPerBlock = 256 / kWave;
         __shared__ double part[kWavesPerBlock];

@@ -404,7 +404,7 @@ int size_fused(tsmi_ctx *c) {
 }
 
 // Two real-time steps in one pass over the whole allocation (fused_kernel.hpp).
-static_assert(kBlock == kStep2Block, "step2_kernel's workgroup size is its launch bound");
+static_assert(kBlock == kStep2Block && kBlock == kStepBlock, "the step kernels' workgroup size is their launch bound");
 template <typename T, int VEC, bool LHY, int MODE, bool PH>
 int launch_step2(tsmi_ctx *c) {
     if (int st = size_fused<T, VEC, LHY, MODE, PH>(c)) return st;
