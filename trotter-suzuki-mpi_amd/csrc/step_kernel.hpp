@@ -635,12 +635,14 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
 #ifndef TSMI_STEP_SYNC
 #define TSMI_STEP_SYNC 4
 #endif
-// Beats between workgroup barriers in step_wave's all-fast loop (FP32 real
-// time; 0: none; a multiple of its 4 beats per iteration): the workgroup's four
-// waves run neighbouring strips, which read the same apron columns, and in step
-// they fetch those lines through L2 once.  8192^2 FP32 per launch: 308.7 ->
-// 299.8 us (every 12 and 24 beats: 301.0, 302.3 us); FP64 564.4 -> 566.9 us with
-// it, so it keeps none.
+// Beats between workgroup barriers in step_wave's all-fast loop (FP32; 0: none;
+// a multiple of its 4 beats per iteration): the workgroup's four waves run
+// neighbouring strips, which read the same apron columns, and in step they
+// fetch those lines through L2 once.  8192^2 FP32 real time per launch: 308.7
+// -> 299.8 us (every 12 and 24 beats: 301.0, 302.3 us); 2048^2 imaginary time
+// 26.22 -> 25.88 us (the workgroup's closing barrier for the partial sums then
+// follows the same number of barriers in each of its waves); FP64 real time
+// 564.4 -> 566.9 us with it, so FP64 keeps none.
 constexpr int kStepSync = TSMI_STEP_SYNC;
 static_assert(kStepSync % 4 == 0, "the all-fast loop runs four beats per iteration");
 constexpr int kStepBlock = 256;      // threads per step_kernel workgroup (its launch bound)
@@ -922,14 +924,14 @@ __device__ __forceinline__ void step_wave(const StepParams<T> &p, const int wave
         // real time, all four waves of the workgroup here (the same segment
         // height, so the same trip count): a barrier every kStepSync beats
         bool sync = false;
-        if constexpr (!IMAG && sizeof(T) == 4 && kStepSync > 0) {
+        if constexpr (sizeof(T) == 4 && kStepSync > 0) {
             sync = block_wave0 >= 0;
 #pragma unroll
             for (int i = 0; i < kStepBlock / kWave; ++i) sync = sync && all_fast_of(block_wave0 + i);
         }
         for (int it = 0; t < t_end; t += 4, ++it) {
             group_fast(t);
-            if constexpr (!IMAG && sizeof(T) == 4 && kStepSync > 0) {
+            if constexpr (sizeof(T) == 4 && kStepSync > 0) {
                 if (sync && it % (kStepSync / 4) == kStepSync / 4 - 1) __builtin_amdgcn_s_barrier();
             }
         }
@@ -972,7 +974,7 @@ __global__ void __launch_bounds__(kStepBlock) step_kernel(StepParams<T> p) {
         constexpr int kWavesPerBlock = 256 / kWave;
         __shared__ double part[kWavesPerBlock];
         double s = 0.0;
-        if (wave < p.n_waves) step_wave<T, IMAG, VEC, LHY, MODE, PH, true>(p, wave, lane, &s);
+        if (wave < p.n_waves) step_wave<T, IMAG, VEC, LHY, MODE, PH, true>(p, wave, lane, &s, wave0);
         if (lane == 0) part[threadIdx.x / kWave] = s;
         __syncthreads();
         if (threadIdx.x == 0) {
